@@ -1,0 +1,161 @@
+/*
+ * fgpe.h -- C ABI of libfgpe, the MI355X (gfx950) split-step Fourier propagator
+ * for the coupled Gross-Pitaevskii equations of spin-orbit-coupled spin-1 and
+ * spin-2 condensates in q1D, q2D and 3D.
+ *
+ * The reference (FORTRESS) has no library interface: its boundary is the
+ * executable cgpe.x, whose time loop calls argument-less subroutines acting on
+ * module globals (SRC/cgpe3d.f90:123-204 declares them; the loop body is
+ * SRC/cgpe3d.f90:451-567).  Each entry point below replaces one group of those
+ * calls; the comment above each declaration cites the reference code it
+ * stands for.  The project's Fortran host (fortress_amd/host/cgpe.f90) binds
+ * these through ISO_C_BINDING and keeps cgpe.x's INPUT/OUTPUT contract.
+ *
+ * Conventions
+ *  - Plain C types only.  Every call returns 0 on success or a negative
+ *    FGPE_E* code; fgpe_last_error() then gives a message.  Nothing throws.
+ *  - Wave functions cross the boundary in the reference's Fortran layout
+ *    PHI(NX,NY,NZ,MS) (SRC/cgpe3d.f90:82,637): x fastest, component slowest,
+ *    each element an interleaved (re, im) pair of doubles.  Unused axes have
+ *    extent 1.
+ *  - A handle owns its device buffers and its HIP stream; calls on one handle
+ *    are not re-entrant.  Different handles are independent.
+ *  - Step semantics are the reference's: first-order Lie splitting
+ *    FFT, KE, SOC, BFT, SE, SP, then NORMC and (imaginary time) the
+ *    projection/NORMT renormalisation (SRC/cgpe3d.f90:452-495).
+ */
+#ifndef FGPE_H
+#define FGPE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define FGPE_OK 0
+#define FGPE_EINVAL (-1)   /* bad argument or unsupported configuration   */
+#define FGPE_EDEVICE (-2)  /* HIP runtime error (allocation, launch, ...) */
+#define FGPE_ENEWTON (-3)  /* spin-2 SOC=0 projection Newton did not converge */
+#define FGPE_ESTATE (-4)   /* call not valid in the handle's current state */
+
+/* precision of the propagation (reductions are always FP64) */
+#define FGPE_FP64 0
+#define FGPE_FP32 1
+
+/* Physical input, one field per record of INPUT/input{1,2,3}D.dat
+ * (SRC/cgpe3d.f90:239-266; 2D SRC/cgpe2d.f90:234-261; 1D SRC/cgpe1d.f90:233-260). */
+typedef struct fgpe_input {
+  int32_t dim;        /* 1, 2 or 3 */
+  int32_t spin;       /* SPIN: 1 or 2 */
+  int32_t niter;      /* NITER */
+  int32_t stp;        /* STP */
+  int32_t nstp;       /* NSTP */
+  int32_t threads;    /* NUMBER OF THREADS (echoed only) */
+  int32_t n[3];       /* NX, NY, NZ (unused axes 1) */
+  double d[3];        /* DX, DY, DZ */
+  double dt;          /* DT */
+  double mass;        /* mass in amu */
+  double a[3];        /* A0, A2, A4 in Bohr radii */
+  int32_t natoms;     /* NATOMS */
+  double nu[3];       /* NUX, NUY, NUZ in Hz */
+  double tol;         /* TOL */
+  int32_t switch_im;  /* SWITCH_IM: 1 imaginary, 0 real time */
+  double gamma[3];    /* GAMMAX, GAMMAY, GAMMAZ */
+  int32_t option_fpc; /* OPTION_FPC (4 = Gaussian) */
+  double mag;         /* MAG */
+} fgpe_input;
+
+/* Derived, dimensionless solver parameters (what the propagator needs). */
+typedef struct fgpe_params {
+  int32_t dim;        /* 1, 2 or 3 */
+  int32_t spin;       /* 1 or 2 */
+  int32_t n[3];       /* grid points per axis (unused axes 1); each even */
+  int32_t imag;       /* 1 imaginary time (CDT = -i DT), 0 real time (CDT = DT) */
+  int32_t precision;  /* FGPE_FP64 or FGPE_FP32 */
+  int32_t option_fpc; /* initial-guess selector used by fgpe_initial_guess */
+  int32_t soc;        /* SWITCH_SOC: 1 if any |gamma| > 1e-40 */
+  double d[3];        /* DX, DY, DZ */
+  double dt;          /* DT */
+  double tau[3];      /* TAU(0..2), the reference's scaled couplings */
+  double alpha[3];    /* ALPHAX, ALPHAY, ALPHAZ = NU/NUX */
+  double gamma[3];    /* GAMMAX, GAMMAY, GAMMAZ */
+  double mag;         /* MAG (used by the SOC = 0 projection) */
+} fgpe_params;
+
+typedef struct fgpe_handle fgpe_handle;
+
+/* Setup arithmetic of the driver: ALPHA = NU/NUX, AOSC, TAU(0..2) per spin and
+ * dimension, SWITCH_SOC.  Replaces SRC/cgpe3d.f90:256,277-283,334-345
+ * (2D SRC/cgpe2d.f90:326-336; 1D SRC/cgpe1d.f90:321-332).
+ * precision selects FGPE_FP64 or FGPE_FP32 for the propagation. */
+int fgpe_derive(const fgpe_input* in, int32_t precision, fgpe_params* out, double* aosc);
+
+/* Allocate device state and tables for one run.  Replaces ALLOCATE_MEM and the
+ * grid/k/potential part of INITIALIZE plus CREATE_PLANS
+ * (SRC/cgpe3d.f90:622-643, 666-754, 843-854). */
+int fgpe_create(const fgpe_params* p, fgpe_handle** h);
+
+/* Release everything.  Replaces DESTROY_PLANS + DEALLOCATE_MEM
+ * (SRC/cgpe3d.f90:859-869, 648-660). */
+void fgpe_destroy(fgpe_handle* h);
+
+/* The initial-guess part of INITIALIZE (Gaussian default and the OPTION_FPC
+ * 1-3 guesses), evaluated on the device (SRC/cgpe3d.f90:753-837;
+ * 2D SRC/cgpe2d.f90:697-774; 1D SRC/cgpe1d.f90:707-789). */
+int fgpe_initial_guess(fgpe_handle* h);
+
+/* Host -> device / device -> host copies of PHI in the Fortran layout
+ * (interleaved complex double).  Upload replaces the initial_sol read
+ * (SRC/cgpe3d.f90:391-409); download serves the dumps (534-565, 596-602). */
+int fgpe_upload(fgpe_handle* h, const double* phi);
+int fgpe_download(fgpe_handle* h, double* phi);
+
+/* NORMT: normalise the total density to 1 and return TNORM
+ * (SRC/cgpe3d.f90:1178-1219; called at 412 before the loop). */
+int fgpe_normalize(fgpe_handle* h, double* tnorm);
+
+/* Advance nsteps full steps of the loop body SRC/cgpe3d.f90:452-495:
+ * FFT, KE, SOC, BFT, SE, SP, NORMC and, in imaginary time, the SOC = 0
+ * projection (spin-1 sigma, spin-2 NEWTON) and NORMT.  After the call the
+ * wave function is in real space and normalised as after line 495. */
+int fgpe_run(fgpe_handle* h, int64_t nsteps);
+
+/* NORMC of the most recent step (taken before renormalisation, as the
+ * reference prints it: SRC/cgpe3d.f90:459,449,527). */
+int fgpe_last_norms(fgpe_handle* h, double* norm);
+
+/* NORMC of the current state (SRC/cgpe3d.f90:1129-1173). */
+int fgpe_normc(fgpe_handle* h, double* norm);
+
+/* ENERGY and RAD of the current state: EN, MU(1..MS), MZ, RMS(1..MS)
+ * (SRC/cgpe3d.f90:1273-1511, 1224-1268).  Energies are returned in the
+ * reference's scaled units (the files print EN/2 and MU/2). */
+int fgpe_observables(fgpe_handle* h, double* en, double* mu, double* mz, double* rms);
+
+/* PHI_OLD handling for the convergence test: keep a copy of the current state
+ * (SRC/cgpe3d.f90:530) and return max |PHI - PHI_OLD| over all points and
+ * components (SRC/cgpe3d.f90:503, before the division by STP*2*DT). */
+int fgpe_keep_old(fgpe_handle* h);
+int fgpe_conv_max(fgpe_handle* h, double* maxdiff);
+
+/* |PHI(NX/2+1,NY/2+1,NZ/2+1,m)| for the file1 rows (SRC/cgpe3d.f90:430,533,579). */
+int fgpe_center_abs(fgpe_handle* h, double* absphi);
+
+/* Wait for all work queued on the handle's stream. */
+int fgpe_sync(fgpe_handle* h);
+
+/* Device time in milliseconds of the last fgpe_run call, measured with HIP
+ * events on the handle's stream (no reference counterpart; the reference
+ * only times the whole program, SRC/cgpe3d.f90:218,606-614). */
+int fgpe_last_run_ms(fgpe_handle* h, double* ms);
+
+/* Message of the last failing call on this thread. */
+int fgpe_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FGPE_H */

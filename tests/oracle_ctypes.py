@@ -1,0 +1,150 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests, the smoke check and the CPU-baseline leg of
+the benchmark load it.  It is the checker, never the thing measured.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from fortress_amd.abi import FgpeInput, FgpeParams
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+        L.or_derive.argtypes = [C.POINTER(FgpeInput), C.POINTER(FgpeParams), dp]
+        L.or_create.argtypes = [C.POINTER(FgpeParams)]
+        L.or_create.restype = vp
+        L.or_destroy.argtypes = [vp]
+        L.or_initial_guess.argtypes = [vp]
+        L.or_set_phi.argtypes = [vp, dp]
+        L.or_get_phi.argtypes = [vp, dp]
+        L.or_normt.argtypes = [vp]
+        L.or_normt.restype = C.c_double
+        L.or_run.argtypes = [vp, C.c_long]
+        L.or_run.restype = ip
+        L.or_last_norms.argtypes = [vp, dp]
+        L.or_normc.argtypes = [vp, dp]
+        L.or_energy.argtypes = [vp, dp, dp, dp]
+        L.or_rad.argtypes = [vp, dp]
+        L.or_set_threads.argtypes = [ip]
+        L.or_simpson.argtypes = [dp, ip, C.c_double]
+        L.or_simpson.restype = C.c_double
+        L.or_diff.argtypes = [dp, ip, C.c_double, dp]
+        L.or_dft.argtypes = [dp, ip, ip]
+        for f in ("or_fft_forward", "or_fft_backward", "or_ke", "or_soc", "or_se", "or_sp"):
+            getattr(L, f).argtypes = [vp]
+        L.or_expm_herm5.argtypes = [dp, C.c_double, C.c_double, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def simpson(f, dx):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().or_simpson(_dp(f), len(f), dx)
+
+
+def diff(p, dx):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty_like(p)
+    lib().or_diff(_dp(p), len(p), dx, _dp(out))
+    return out
+
+
+def dft(z, sign):
+    a = np.ascontiguousarray(np.asarray(z, dtype=np.complex128)).copy()
+    lib().or_dft(_dp(a.view(np.float64)), len(a), sign)
+    return a
+
+
+def expm_herm5(h, cdt, v):
+    h = np.ascontiguousarray(h, dtype=np.complex128)
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    out = np.empty(5, dtype=np.complex128)
+    lib().or_expm_herm5(_dp(h.view(np.float64)), cdt.real, cdt.imag, _dp(v.view(np.float64)), _dp(out.view(np.float64)))
+    return out
+
+
+def derive(inp):
+    p = FgpeParams()
+    aosc = C.c_double()
+    lib().or_derive(C.byref(inp), C.byref(p), C.byref(aosc))
+    return p, aosc.value
+
+
+class Oracle:
+    """FP64 restatement of one cgpe.x run; phi in Fortran order (ms, nz, ny, nx)."""
+
+    def __init__(self, params):
+        self.p = params
+        self.ms = 2 * params.spin + 1
+        self.shape = (self.ms, params.n[2], params.n[1], params.n[0])
+        self.h = lib().or_create(C.byref(params))
+
+    def close(self):
+        if self.h:
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def initial_guess(self):
+        lib().or_initial_guess(self.h)
+
+    def set_phi(self, phi):
+        a = np.ascontiguousarray(phi, dtype=np.complex128).reshape(self.shape)
+        lib().or_set_phi(self.h, _dp(a.view(np.float64)))
+
+    def get_phi(self):
+        a = np.empty(self.shape, dtype=np.complex128)
+        lib().or_get_phi(self.h, _dp(a.view(np.float64)))
+        return a
+
+    def normt(self):
+        return lib().or_normt(self.h)
+
+    def run(self, n):
+        rc = lib().or_run(self.h, n)
+        if rc:
+            raise RuntimeError("oracle step failed (%d)" % rc)
+
+    def last_norms(self):
+        a = np.empty(self.ms)
+        lib().or_last_norms(self.h, _dp(a))
+        return a
+
+    def normc(self):
+        a = np.empty(self.ms)
+        lib().or_normc(self.h, _dp(a))
+        return a
+
+    def energy(self):
+        mu = np.empty(self.ms)
+        en = C.c_double()
+        mz = C.c_double()
+        lib().or_energy(self.h, _dp(mu), C.byref(en), C.byref(mz))
+        return en.value, mu, mz.value
+
+    def rad(self):
+        a = np.empty(self.ms)
+        lib().or_rad(self.h, _dp(a))
+        return a
+
+    def call(self, name):
+        getattr(lib(), name)(self.h)
