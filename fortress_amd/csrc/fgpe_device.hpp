@@ -1,0 +1,550 @@
+// fgpe_device.hpp -- device building blocks of libfgpe for gfx950 (CDNA4).
+//
+//  * complex arithmetic on float2/double2 (interleaved re/im, 8/16-byte loads)
+//  * in-register small DFTs (radix 2, 3, 4, 6, 8, 12, 16)
+//  * a Stockham line FFT in which each of the P = N/Q threads of a line keeps
+//    Q points in registers and exchanges through LDS between stages
+//  * the pointwise physics of one grid point (SE, SP, SOC) in registers
+//
+// The FFT replaces FFTW's fftw_execute_dft (SRC/cgpe3d.f90:1760,1781): an
+// unnormalised DFT, forward sign -1, backward +1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fgpe {
+
+template <typename T> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+template <typename T> using C_t = typename cplx_of<T>::type;
+
+template <typename C> __host__ __device__ __forceinline__ C mk(decltype(C::x) re, decltype(C::x) im) {
+  C c;
+  c.x = re;
+  c.y = im;
+  return c;
+}
+template <typename C> __host__ __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __host__ __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __host__ __device__ __forceinline__ C cmul(C a, C b) {
+  return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename C> __host__ __device__ __forceinline__ C cmulc(C a, C b) {
+  return mk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename C> __host__ __device__ __forceinline__ C cconj(C a) { return mk<C>(a.x, -a.y); }
+template <typename C, typename T> __host__ __device__ __forceinline__ C cscale(C a, T s) { return mk<C>(a.x * s, a.y * s); }
+// i*a and -i*a
+template <typename C> __host__ __device__ __forceinline__ C cmuli(C a) { return mk<C>(-a.y, a.x); }
+template <typename C> __host__ __device__ __forceinline__ C cmulmi(C a) { return mk<C>(a.y, -a.x); }
+template <typename C> __host__ __device__ __forceinline__ decltype(C::x) cabs2(C a) { return a.x * a.x + a.y * a.y; }
+
+// ---------------------------------------------------------------- small DFTs
+// Exact twiddle constants omega_R^e = cos(2 pi e/R) + DIR i sin(2 pi e/R).
+__host__ __device__ constexpr double cos_tab(int e, int R) {
+  // first-octant values, extended by symmetry
+  constexpr double C16_1 = 0.92387953251128675613, S16_1 = 0.38268343236508977173;
+  constexpr double H = 0.70710678118654752440, R3 = 0.86602540378443864676;
+  e = ((e % R) + R) % R;
+  if (R == 16) {
+    constexpr double t[16] = {1, C16_1, H, S16_1, 0, -S16_1, -H, -C16_1, -1, -C16_1, -H, -S16_1, 0, S16_1, H, C16_1};
+    return t[e];
+  }
+  if (R == 12) {
+    constexpr double t[12] = {1, R3, 0.5, 0, -0.5, -R3, -1, -R3, -0.5, 0, 0.5, R3};
+    return t[e];
+  }
+  if (R == 8) {
+    constexpr double t[8] = {1, H, 0, -H, -1, -H, 0, H};
+    return t[e];
+  }
+  if (R == 6) {
+    constexpr double t[6] = {1, 0.5, -0.5, -1, -0.5, 0.5};
+    return t[e];
+  }
+  if (R == 4) {
+    constexpr double t[4] = {1, 0, -1, 0};
+    return t[e];
+  }
+  if (R == 3) {
+    constexpr double t[3] = {1, -0.5, -0.5};
+    return t[e];
+  }
+  constexpr double t2[2] = {1, -1};
+  return t2[e];
+}
+__host__ __device__ constexpr double sin_tab(int e, int R) {
+  constexpr double C16_1 = 0.92387953251128675613, S16_1 = 0.38268343236508977173;
+  constexpr double H = 0.70710678118654752440, R3 = 0.86602540378443864676;
+  e = ((e % R) + R) % R;
+  if (R == 16) {
+    constexpr double t[16] = {0, S16_1, H, C16_1, 1, C16_1, H, S16_1, 0, -S16_1, -H, -C16_1, -1, -C16_1, -H, -S16_1};
+    return t[e];
+  }
+  if (R == 12) {
+    constexpr double t[12] = {0, 0.5, R3, 1, R3, 0.5, 0, -0.5, -R3, -1, -R3, -0.5};
+    return t[e];
+  }
+  if (R == 8) {
+    constexpr double t[8] = {0, H, 1, H, 0, -H, -1, -H};
+    return t[e];
+  }
+  if (R == 6) {
+    constexpr double t[6] = {0, R3, R3, 0, -R3, -R3};
+    return t[e];
+  }
+  if (R == 4) {
+    constexpr double t[4] = {0, 1, 0, -1};
+    return t[e];
+  }
+  if (R == 3) {
+    constexpr double t[3] = {0, R3, -R3};
+    return t[e];
+  }
+  return 0.0;
+}
+
+// v *= omega_R^e (sign DIR), e a compile-time constant after unrolling
+template <int R, int DIR, typename C> __host__ __device__ __forceinline__ C twc(C v, int e) {
+  using T = decltype(C::x);
+  e = ((e % R) + R) % R;
+  if (e == 0) return v;
+  if (4 * e == R) return DIR > 0 ? cmuli(v) : cmulmi(v);
+  if (2 * e == R) return mk<C>(-v.x, -v.y);
+  if (4 * e == 3 * R) return DIR > 0 ? cmulmi(v) : cmuli(v);
+  const T c = (T)cos_tab(e, R);
+  const T s = (T)(DIR * sin_tab(e, R));
+  return mk<C>(v.x * c - v.y * s, v.x * s + v.y * c);
+}
+
+template <int R, int DIR, typename C> struct Dft;
+
+template <int DIR, typename C> struct Dft<2, DIR, C> {
+  __host__ __device__ __forceinline__ static void run(C* v) {
+    C a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+  }
+};
+template <int DIR, typename C> struct Dft<3, DIR, C> {
+  __host__ __device__ __forceinline__ static void run(C* v) {
+    using T = decltype(C::x);
+    const T h = (T)0.86602540378443864676;
+    C s = cadd(v[1], v[2]);
+    C d = csub(v[1], v[2]);
+    C a = v[0];
+    v[0] = cadd(a, s);
+    C m = mk<C>(a.x - (T)0.5 * s.x, a.y - (T)0.5 * s.y);
+    // DIR * i * h * d
+    C t = DIR > 0 ? mk<C>(-h * d.y, h * d.x) : mk<C>(h * d.y, -h * d.x);
+    v[1] = cadd(m, t);
+    v[2] = csub(m, t);
+  }
+};
+template <int DIR, typename C> struct Dft<4, DIR, C> {
+  __host__ __device__ __forceinline__ static void run(C* v) {
+    C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+    C id13 = DIR > 0 ? cmuli(d13) : cmulmi(d13);  // DIR*i*d13
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, id13);
+    v[3] = csub(d02, id13);
+  }
+};
+// composite R = A*B:  n = A*n1 + n2, k = k1 + B*k2
+template <int A, int B, int DIR, typename C> struct DftAB {
+  __host__ __device__ __forceinline__ static void run(C* v) {
+    constexpr int R = A * B;
+    C y[A][B];
+#pragma unroll
+    for (int n2 = 0; n2 < A; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < B; ++n1) y[n2][n1] = v[A * n1 + n2];
+      Dft<B, DIR, C>::run(y[n2]);
+#pragma unroll
+      for (int k1 = 0; k1 < B; ++k1) y[n2][k1] = twc<R, DIR>(y[n2][k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < B; ++k1) {
+      C z[A];
+#pragma unroll
+      for (int n2 = 0; n2 < A; ++n2) z[n2] = y[n2][k1];
+      Dft<A, DIR, C>::run(z);
+#pragma unroll
+      for (int k2 = 0; k2 < A; ++k2) v[k1 + B * k2] = z[k2];
+    }
+  }
+};
+template <int DIR, typename C> struct Dft<6, DIR, C> : DftAB<2, 3, DIR, C> {};
+template <int DIR, typename C> struct Dft<8, DIR, C> : DftAB<2, 4, DIR, C> {};
+template <int DIR, typename C> struct Dft<12, DIR, C> : DftAB<4, 3, DIR, C> {};
+template <int DIR, typename C> struct Dft<16, DIR, C> : DftAB<4, 4, DIR, C> {};
+
+// ------------------------------------------------------------ line FFT plan
+// radix of the stage that leaves `rem` points to combine, given Q points per
+// thread: the largest of 16,12,8,6,4,3,2 dividing both.
+__host__ __device__ constexpr int pick_radix(int rem, int q) {
+  return (rem % 16 == 0 && q % 16 == 0) ? 16
+         : (rem % 12 == 0 && q % 12 == 0) ? 12
+         : (rem % 8 == 0 && q % 8 == 0)   ? 8
+         : (rem % 6 == 0 && q % 6 == 0)   ? 6
+         : (rem % 4 == 0 && q % 4 == 0)   ? 4
+         : (rem % 3 == 0 && q % 3 == 0)   ? 3
+         : (rem % 2 == 0 && q % 2 == 0)   ? 2
+                                          : 0;
+}
+// total size of the per-stage twiddle tables (stages with NS > 1)
+__host__ __device__ constexpr int twiddle_size(int n, int q) {
+  int ns = 1, sz = 0;
+  while (ns < n) {
+    int r = pick_radix(n / ns, q);
+    if (r == 0) return -1;
+    if (ns > 1) sz += ns * r;
+    ns *= r;
+  }
+  return sz;
+}
+// LDS index padding: one spare slot every 16 complex to spread banks
+__host__ __device__ constexpr int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int line_stride(int n) { return lpad(n) + 1; }
+
+// One Stockham stage.  Thread j of the line owns butterflies b = j + u*P,
+// u < Q/R; its inputs are always its registers v[u + r*Q/R] (positions
+// j + P*(u + r*Q/R)), so registers hold positions j + P*r' before every stage
+// and after the last one.  Intermediate outputs go through LDS.
+template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
+  static constexpr int R = pick_radix(N / NS, Q);
+  static constexpr int P = N / Q;
+  static constexpr bool LAST = (NS * R == N);
+  static_assert(R > 0, "unsupported FFT size / points-per-thread");
+  __device__ __forceinline__ static void run(C (&v)[Q], C* lds, const C* tw, int j) {
+    C x[Q];
+#pragma unroll
+    for (int u = 0; u < Q / R; ++u) {
+      const int b = j + u * P;
+      const int k = b % NS;
+      C y[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[r] = v[u + r * (Q / R)];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C w = tw[r * NS + k];
+          if constexpr (DIR > 0) w.y = -w.y;
+          y[r] = cmul(y[r], w);
+        }
+      }
+      Dft<R, DIR, C>::run(y);
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[u + r * (Q / R)] = y[r];
+    }
+    if constexpr (LAST) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[q] = x[q];
+    } else {
+      __syncthreads();  // previous readers of this LDS line are done
+#pragma unroll
+      for (int u = 0; u < Q / R; ++u) {
+        const int b = j + u * P;
+        const int k = b % NS;
+        const int idxd = (b / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lpad(idxd + r * NS)] = x[u + r * (Q / R)];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[q] = lds[lpad(j + P * q)];
+      FftStage<N, Q, NS * R, DIR, C>::run(v, lds, tw + (NS > 1 ? NS * R : 0), j);
+    }
+  }
+};
+
+// In-place line FFT of N points held as v[q] = x[j + P*q] by the P threads
+// of a line; afterwards v[q] = X[j + P*q].  lds: this line's LDS scratch
+// (line_stride(N) elements); tw: the per-stage twiddle tables.
+template <int N, int Q, int DIR, typename C>
+__device__ __forceinline__ void line_fft(C (&v)[Q], C* lds, const C* tw, int j) {
+  FftStage<N, Q, 1, DIR, C>::run(v, lds, tw, j);
+}
+
+// ---------------------------------------------------------------- physics
+// Real-valued parameters of one run, passed by value to every kernel.
+struct Phys {
+  double tau0, tau1, tau2;
+  double dt;
+  double gx, gy, gz;
+  double inv_npts;  // 1/(NX*NY*NZ), BFT normalisation (SRC/cgpe3d.f90:1782)
+  int imag;         // CDT = -i DT (1) or DT (0)
+  int dim;
+};
+
+__host__ __device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
+__host__ __device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
+
+// exp(-i CDT e) for real e: real time exp(-i DT e), imaginary time exp(-DT e)
+template <typename C, typename T> __host__ __device__ __forceinline__ C phase_of(T e, T dt, int imag) {
+  if (imag) return mk<C>(exp(-dt * e), (T)0);
+  T s, c;
+  sincos_t(dt * e, &s, &c);
+  return mk<C>(c, -s);
+}
+
+// (-i CDT) * z :  real time -i DT z, imaginary time -DT z
+template <typename C, typename T> __host__ __device__ __forceinline__ C mul_micdt(C z, T dt, int imag) {
+  return imag ? mk<C>(-dt * z.x, -dt * z.y) : mk<C>(dt * z.y, -dt * z.x);
+}
+
+// SP: diagonal trap + density phases (SRC/cgpe3d.f90:1023-1063)
+template <int SPIN, typename C, typename T>
+__host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T dt) {
+  const T t0 = (T)p.tau0, t1 = (T)p.tau1, t2 = (T)p.tau2;
+  if constexpr (SPIN == 1) {
+    T a0 = cabs2(ph[0]), a1 = cabs2(ph[1]), a2 = cabs2(ph[2]);
+    T rho = a0 + a1 + a2;
+    T base = v + t0 * rho;
+    T e[3] = {base + t1 * (a0 + a1 - a2), base + t1 * (a0 + a2), base + t1 * (a2 + a1 - a0)};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) ph[m] = cmul(ph[m], phase_of<C, T>(e[m], dt, p.imag));
+  } else {
+    T a[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a[m] = cabs2(ph[m]);
+    T rho = a[0] + a[1] + a[2] + a[3] + a[4];
+    T fz = (T)2 * a[0] + a[1] - a[3] - (T)2 * a[4];
+    T base = v + t0 * rho;
+    T e[5] = {base + (T)2 * t1 * fz + (T)0.4 * t2 * a[4], base + t1 * fz + (T)0.4 * t2 * a[3],
+              base + (T)0.2 * t2 * a[2], base - t1 * fz + (T)0.4 * t2 * a[1],
+              base - (T)2 * t1 * fz + (T)0.4 * t2 * a[0]};
+#pragma unroll
+    for (int m = 0; m < 5; ++m) ph[m] = cmul(ph[m], phase_of<C, T>(e[m], dt, p.imag));
+  }
+}
+
+// F_x - i F_y and F_z of a spin-2 spinor with the reference's SX, SY, SZ
+// (SRC/cgpe3d.f90:686-696,1086-1088), then C12..C45 (SRC/cgpe3d.f90:1112-1122)
+template <typename C, typename T> __host__ __device__ __forceinline__ void mat_c2(const C* ph, const Phys& p, C* c) {
+  const T r = (T)1.22474487139158904910;  // sqrt(1.5)
+  // F_- = sum conj(phi) (SX - i SY) phi  restricted to its real parts:
+  // FX = Re(phi^H SX phi), FY = Re(phi^H SY phi)
+  C sx[5] = {ph[1], cadd(ph[0], cscale(ph[2], r)), cscale(cadd(ph[1], ph[3]), r), cadd(cscale(ph[2], r), ph[4]), ph[3]};
+  T fx = 0, fy = 0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) fx += ph[m].x * sx[m].x + ph[m].y * sx[m].y;
+  // SY phi = i * {-p1, p0 - r p2, r p1 - r p3, r p2 - p4, p3}
+  C sy[5] = {cmuli(mk<C>(-ph[1].x, -ph[1].y)), cmuli(csub(ph[0], cscale(ph[2], r))),
+             cmuli(cscale(csub(ph[1], ph[3]), r)), cmuli(csub(cscale(ph[2], r), ph[4])), cmuli(ph[3])};
+#pragma unroll
+  for (int m = 0; m < 5; ++m) fy += ph[m].x * sy[m].x + ph[m].y * sy[m].y;
+  const T t1 = (T)p.tau1, t2 = (T)p.tau2;
+  C fm = mk<C>(t1 * fx, -t1 * fy);  // tau1 * (FX - i FY)
+  c[0] = csub(fm, cscale(cmulc(ph[3], ph[4]), (T)0.4 * t2));
+  c[1] = cscale(cmulc(ph[2], ph[4]), (T)0.2 * t2);
+  c[2] = csub(cscale(fm, r), cscale(cmulc(ph[2], ph[3]), (T)0.2 * t2));
+  c[3] = csub(cscale(fm, r), cscale(cmulc(ph[1], ph[2]), (T)0.2 * t2));
+  c[4] = cscale(cmulc(ph[0], ph[2]), (T)0.2 * t2);
+  c[5] = csub(fm, cscale(cmulc(ph[0], ph[1]), (T)0.4 * t2));
+}
+
+// Hbar v with Hbar(1,2)=C12,(1,3)=C13,(2,3)=C23,(3,4)=C34,(3,5)=C35,(4,5)=C45
+template <typename C> __host__ __device__ __forceinline__ void hmul2(const C* c, const C* v, C* o) {
+  o[0] = cadd(cmul(c[0], v[1]), cmul(c[1], v[2]));
+  o[1] = cadd(cmulc(v[0], c[0]), cmul(c[2], v[2]));
+  o[2] = cadd(cadd(cmulc(v[0], c[1]), cmulc(v[1], c[2])), cadd(cmul(c[3], v[3]), cmul(c[4], v[4])));
+  o[3] = cadd(cmulc(v[2], c[3]), cmul(c[5], v[4]));
+  o[4] = cadd(cmulc(v[2], c[4]), cmulc(v[3], c[5]));
+}
+
+// SE spin-2 (SRC/cgpe3d.f90:1673-1741): predictor with factor cpred (1/2 in
+// 1D, 1 in 2D/3D), averaged couplings, then exp(-i CDT Hbar) phi.  ZHEEV's
+// V diag(exp(-i CDT w)) V^H is replaced by its Taylor series applied to the
+// vector, summed to rounding (the matrix exponential is unique); when
+// |CDT|*||Hbar||_inf exceeds 1/2 the exponential is split into equal
+// sub-steps.
+template <typename C, typename T>
+__host__ __device__ __forceinline__ void se2_point(C* ph, const Phys& p, T dt, T cpred, T tol) {
+  C c0[6], c1[6], hv[5], p1[5];
+  mat_c2<C, T>(ph, p, c0);
+  hmul2(c0, ph, hv);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) p1[m] = cadd(ph[m], mul_micdt<C, T>(hv[m], dt * cpred, p.imag));
+  mat_c2<C, T>(p1, p, c1);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) c0[i] = cscale(cadd(c0[i], c1[i]), (T)0.5);
+  // infinity-norm bound with |re|+|im|
+  T n[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) n[i] = fabs(c0[i].x) + fabs(c0[i].y);
+  T rows = fmax(fmax(n[0] + n[1], n[0] + n[2]), fmax(n[1] + n[2] + n[3] + n[4], fmax(n[3] + n[5], n[4] + n[5])));
+  T b = dt * rows;
+  // bounded sub-step count (a non-finite bound must not spin the loop)
+  int nsub = b > (T)0.5 ? (b < (T)32 ? (int)ceil(b / (T)0.5) : 64) : 1;
+  T h = dt / (T)nsub;
+  for (int s = 0; s < nsub; ++s) {
+    C term[5], sum[5];
+    T smax = 0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      term[m] = ph[m];
+      sum[m] = ph[m];
+      smax = fmax(smax, fabs(ph[m].x) + fabs(ph[m].y));
+    }
+    for (int k = 1; k < 40; ++k) {
+      hmul2(c0, term, hv);
+      T f = h / (T)k;
+      T tmax = 0;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        term[m] = mul_micdt<C, T>(hv[m], f, p.imag);
+        sum[m] = cadd(sum[m], term[m]);
+        tmax = fmax(tmax, fabs(term[m].x) + fabs(term[m].y));
+      }
+      if (tmax <= tol * smax) break;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) ph[m] = sum[m];
+  }
+}
+
+// complex cos and sin of w = a + i b
+template <typename C, typename T> __host__ __device__ __forceinline__ void ccossin(C w, C& cw, C& sw) {
+  T sa, ca;
+  sincos_t(w.x, &sa, &ca);
+  T ch = cosh(w.y), sh = sinh(w.y);
+  cw = mk<C>(ca * ch, -sa * sh);
+  sw = mk<C>(sa * ch, ca * sh);
+}
+template <typename C> __host__ __device__ __forceinline__ C cdiv(C a, C b) {
+  auto d = b.x * b.x + b.y * b.y;
+  return mk<C>((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// SE spin-1, closed form (SRC/cgpe3d.f90:1637-1672).  For |omega| below
+// `small` the functions (cos w - 1)/w^2 and -i sin(w)/w come from their
+// Taylor series (the closed form loses all digits there in FP32); exactly at
+// omega = 0 both forms give delta = -1/2, kappa = -i as the reference does.
+template <typename C, typename T>
+__host__ __device__ __forceinline__ void se1_point(C* ph, const Phys& p, T dt, T small) {
+  const T t1 = (T)p.tau1;
+  C a1 = cscale(cmulc(ph[1], ph[2]), t1);
+  C b1 = cscale(cmulc(ph[1], ph[0]), t1);
+  T om_r = sqrt(cabs2(a1) + cabs2(b1));
+  // CDT: real time (dt,0), imaginary (0,-dt)
+  C cdt = p.imag ? mk<C>((T)0, -dt) : mk<C>(dt, (T)0);
+  C w = cscale(cdt, om_r);
+  C delta, kappa;
+  T aw = sqrt(cabs2(w));
+  if (aw > small) {
+    C cw, sw;
+    ccossin<C, T>(w, cw, sw);
+    C w2 = cmul(w, w);
+    delta = cdiv(mk<C>(cw.x - (T)1, cw.y), w2);
+    kappa = cmulmi(cdiv(sw, w));
+  } else {
+    C w2 = cmul(w, w);
+    C w4 = cmul(w2, w2);
+    C w6 = cmul(w4, w2);
+    delta = mk<C>((T)-0.5 + w2.x / (T)24 - w4.x / (T)720 + w6.x / (T)40320,
+                  w2.y / (T)24 - w4.y / (T)720 + w6.y / (T)40320);
+    C sk = mk<C>((T)1 - w2.x / (T)6 + w4.x / (T)120 - w6.x / (T)5040,
+                 -w2.y / (T)6 + w4.y / (T)120 - w6.y / (T)5040);
+    kappa = cmulmi(sk);
+  }
+  C dc2 = cmul(delta, cmul(cdt, cdt));
+  C kc = cmul(kappa, cdt);
+  C aa = mk<C>(cabs2(a1), (T)0), bb = mk<C>(cabs2(b1), (T)0);
+  C t1v = cadd(cmul(dc2, cadd(cmul(aa, ph[0]), cmul(cmulc(a1, b1), ph[2]))), cmul(kc, cmul(a1, ph[1])));
+  C t2v = cadd(cmul(dc2, cmul(cadd(aa, bb), ph[1])), cmul(kc, cadd(cmulc(ph[0], a1), cmulc(ph[2], b1))));
+  C t3v = cadd(cmul(dc2, cadd(cmul(cmulc(b1, a1), ph[0]), cmul(bb, ph[2]))), cmul(kc, cmul(b1, ph[1])));
+  ph[0] = cadd(ph[0], t1v);
+  ph[1] = cadd(ph[1], t2v);
+  ph[2] = cadd(ph[2], t3v);
+}
+
+// SOC factor A(beta) diag(exp(-i CDT L alpha)) A(beta)^H applied to one
+// k-point (SRC/cgpe3d.f90:904-1004).  A(beta) = E A0 with A0 the real
+// constant matrix of the 1D code (SRC/cgpe1d.f90:627-675) and
+// E = diag(e1^S, ..., e1^-S), e1 = exp(-i beta); so the factor is applied as
+// E, A0, D, A0^T, E^H in that order from the right.  d[c] = exp(-i CDT L alpha)
+// for column c = L + S.
+template <int SPIN, typename C, typename T>
+__host__ __device__ __forceinline__ void soc_apply(C* v, C e1, const C* d) {
+  if constexpr (SPIN == 1) {
+    const T s5 = (T)0.70710678118654752440;
+    // A0 rows: [.5,-s5,.5],[-s5,0,s5],[.5,s5,.5]
+    C e2 = cmul(e1, e1);
+    C w0 = cmulc(v[0], e2), w1 = cmulc(v[1], e1), w2 = v[2];  // E^H v
+    // A0^T w
+    C u0 = cadd(cscale(cadd(w0, w2), (T)0.5), cscale(w1, -s5));
+    C u1 = cscale(csub(w2, w0), s5);
+    C u2 = cadd(cscale(cadd(w0, w2), (T)0.5), cscale(w1, s5));
+    u0 = cmul(u0, d[0]);
+    u1 = cmul(u1, d[1]);
+    u2 = cmul(u2, d[2]);
+    // A0 u
+    C r0 = csub(cscale(cadd(u0, u2), (T)0.5), cscale(u1, s5));
+    C r1 = cscale(csub(u2, u0), s5);
+    C r2 = cadd(cscale(cadd(u0, u2), (T)0.5), cscale(u1, s5));
+    // A0 = [[.5,-s5,.5],[-s5,0,s5],[.5,s5,.5]]:  row0 = .5u0 - s5 u1 + .5u2,
+    // row1 = -s5 u0 + s5 u2, row2 = .5u0 + s5 u1 + .5u2
+    v[0] = cmul(r0, e2);
+    v[1] = cmul(r1, e1);
+    v[2] = r2;
+  } else {
+    const T q = (T)0.61237243569579452455;  // sqrt(0.375)
+    C e2 = cmul(e1, e1);
+    C w0 = cmulc(v[0], e2), w1 = cmulc(v[1], e1), w2 = v[2], w3 = cmul(v[3], e1), w4 = cmul(v[4], e2);
+    // A0 rows (1D constant A): r1 [.25,-.5,q,-.5,.25], r2 [-.5,.5,0,-.5,.5],
+    // r3 [q,0,-.5,0,q], r4 [-.5,-.5,0,.5,.5], r5 [.25,.5,q,.5,.25]
+    // u = A0^T w (u_c = sum_r A0[r][c] w_r)
+    C s04 = cadd(w0, w4), s13 = cadd(w1, w3), d13 = csub(w3, w1), d04 = csub(w4, w0);
+    C u0 = cadd(cadd(cscale(s04, (T)0.25), cscale(s13, (T)-0.5)), cscale(w2, q));
+    C u1 = cadd(cscale(d04, (T)0.5), cscale(csub(w1, w3), (T)0.5));
+    C u2 = cadd(cscale(s04, q), cscale(w2, (T)-0.5));
+    C u3 = cadd(cscale(d04, (T)0.5), cscale(d13, (T)0.5));
+    C u4 = cadd(cadd(cscale(s04, (T)0.25), cscale(s13, (T)0.5)), cscale(w2, q));
+    u0 = cmul(u0, d[0]);
+    u1 = cmul(u1, d[1]);
+    u2 = cmul(u2, d[2]);
+    u3 = cmul(u3, d[3]);
+    u4 = cmul(u4, d[4]);
+    // r = A0 u
+    C r0 = cadd(cadd(cscale(u0, (T)0.25), cscale(cadd(u1, u3), (T)-0.5)), cadd(cscale(u2, q), cscale(u4, (T)0.25)));
+    C r1 = cscale(cadd(csub(u1, u0), csub(u4, u3)), (T)0.5);
+    C r2 = cadd(cscale(cadd(u0, u4), q), cscale(u2, (T)-0.5));
+    C r3 = cscale(cadd(csub(u3, u0), csub(u4, u1)), (T)0.5);
+    C r4 = cadd(cadd(cscale(u0, (T)0.25), cscale(cadd(u1, u3), (T)0.5)), cadd(cscale(u2, q), cscale(u4, (T)0.25)));
+    v[0] = cmul(r0, e2);
+    v[1] = cmul(r1, e1);
+    v[2] = r2;
+    v[3] = cmulc(r3, e1);
+    v[4] = cmulc(r4, e2);
+  }
+}
+
+// e1 = exp(-i beta) and d[L+S] = exp(-i CDT L alpha) for one (kx, ky)
+// (ALPHA, BETA as in SRC/cgpe3d.f90:956-961; beta = 0 when alpha <= 1e-40).
+// In 1D the reference uses the constant basis with exponent 2 L GAMMAX kx
+// (SRC/cgpe1d.f90:1518), i.e. e1 = 1 and a signed alpha = 2 GAMMAX kx.
+template <int SPIN, typename C, typename T>
+__host__ __device__ __forceinline__ void soc_factors(T kx, T ky, const Phys& p, T dt, C& e1, C* d) {
+  T alpha;
+  if (p.dim == 1) {
+    alpha = (T)2 * (T)p.gx * kx;
+    e1 = mk<C>((T)1, (T)0);
+  } else {
+    T ax = (T)p.gx * kx, ay = (T)p.gy * ky;
+    T r = sqrt(ax * ax + ay * ay);
+    alpha = (T)2 * r;
+    if (alpha <= (T)1e-30)
+      e1 = mk<C>((T)1, (T)0);
+    else
+      e1 = mk<C>(ax / r, -ay / r);
+  }
+#pragma unroll
+  for (int L = -SPIN; L <= SPIN; ++L) d[L + SPIN] = phase_of<C, T>((T)L * alpha, dt, p.imag);
+}
+
+}  // namespace fgpe

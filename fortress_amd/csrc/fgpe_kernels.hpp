@@ -1,0 +1,449 @@
+// fgpe_kernels.hpp -- the propagator passes of libfgpe.
+//
+// State layout in HBM: planar per component, phi[m][z][y][x] (x fastest),
+// float2 (FP32 mode) or double2 (FP64 mode).  One time step of the reference
+// loop (SRC/cgpe3d.f90:452-495) becomes line passes that keep whole FFT lines
+// in registers/LDS:
+//   3D: B  forward-y per component (applies the pending renormalisation)
+//       C  forward-z + KE + SOC + inverse-z, all components
+//       D  inverse-y per component
+//       A  inverse-x + 1/N + SE + SP + NORMC partials + forward-x, all comps
+//   2D: Y  forward-y + KE + SOC + inverse-y (all comps), then A
+//   1D: one workgroup runs whole steps (k_line1d)
+// followed by a one-block finalize that turns the NORMC partials into the
+// norms and the per-component scale of the imaginary-time renormalisation.
+#pragma once
+#include "fgpe_device.hpp"
+
+namespace fgpe {
+
+enum : int { XF_INV = 1, XF_PHYS = 2, XF_FWD = 4, XF_SCALE = 8 };
+
+template <typename T> struct Tables {
+  const C_t<T>* twx;  // per-stage FFT twiddles, x axis
+  const C_t<T>* twy;
+  const C_t<T>* twz;
+  const C_t<T>* kea;  // exp(-i CDT kx^2)            [NX]
+  const C_t<T>* keb;  // exp(-i CDT ky^2) (3D), 1     [NY or 1]
+  const C_t<T>* kel;  // line-axis factor per comp    [MS][N_line]
+  const T* kx;        // wave numbers (FFT order, positive Nyquist)
+  const T* ky;
+  const T* vx;  // ALPHAX^2 x^2
+  const T* vy;
+  const T* vz;
+  const double* wx;  // Simpson weights (SRC/simpson.f90)
+  const double* wy;
+  const double* wz;
+  const C_t<T>* cscale;  // pending per-component renormalisation [MS]
+};
+
+struct Geom {
+  int nx, ny, nz;
+  long npts;
+};
+
+// ----------------------------------------------------------- reductions
+// butterfly sum over the active lanes of a wave; blocks narrower than a wave
+// (small 1D lines) only exchange among their own lanes
+template <int M> __device__ __forceinline__ void wave_sum(double (&a)[M]) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    if (off < (int)blockDim.x)
+#pragma unroll
+      for (int m = 0; m < M; ++m) a[m] += __shfl_xor(a[m], off, 64);
+}
+
+// sum a[M] over the block and write out[blockIdx.x*M + m]; red: >= 16*M doubles
+template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M], double* red, double* out) {
+  wave_sum<M>(a);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int m = 0; m < M; ++m) red[wid * M + m] = a[m];
+  __syncthreads();
+  if (threadIdx.x < M) {
+    double s = 0;
+    for (int w = 0; w < nw; ++w) s += red[w * M + threadIdx.x];
+    out[(long)blockIdx.x * M + threadIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------ pass A (x)
+// Lines along x, all components.  flags: XF_INV inverse FFT first, XF_PHYS
+// real-space physics (1/N, SE, SP, NORMC partials), XF_FWD forward FFT last,
+// XF_SCALE multiply by the pending scale on load.
+template <typename T, int SPIN, int N, int Q>
+__global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
+                                               double* __restrict__ partials) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  double* red = reinterpret_cast<double*>(lds_tw + TWS);
+  C* lds = reinterpret_cast<C*>(red + 16 * MS);
+
+  const int lpw = blockDim.x / P;
+  const int tl = threadIdx.x / P, j = threadIdx.x % P;
+  const long line = (long)blockIdx.x * lpw + tl;
+  const long nlines = (long)g.ny * g.nz;
+  const bool valid = line < nlines;
+  const int y = valid ? (int)(line % g.ny) : 0, z = valid ? (int)(line / g.ny) : 0;
+  C* my = lds + tl * LS;
+
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
+
+  C v[MS][Q];
+  const long base = line * N;
+#pragma unroll
+  for (int m = 0; m < MS; ++m) {
+    C sc = mk<C>((T)1, (T)0);
+    if (flags & XF_SCALE) sc = tb.cscale[m];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      C a = valid ? phi[m * g.npts + base + j + P * q] : mk<C>((T)0, (T)0);
+      v[m][q] = (flags & XF_SCALE) ? cmul(a, sc) : a;
+    }
+  }
+  __syncthreads();
+  if (flags & XF_INV) {
+#pragma unroll
+    for (int m = 0; m < MS; ++m) line_fft<N, Q, +1>(v[m], my, lds_tw, j);
+  }
+  if (flags & XF_PHYS) {
+    const T dt = (T)ph.dt;
+    const T inv = (T)ph.inv_npts;
+    const T vyz = tb.vy[y] + tb.vz[z];
+    const double wyz = tb.wy[y] * tb.wz[z];
+    double acc[MS];
+#pragma unroll
+    for (int m = 0; m < MS; ++m) acc[m] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int x = j + P * q;
+      C pt[MS];
+#pragma unroll
+      for (int m = 0; m < MS; ++m) pt[m] = cscale(v[m][q], inv);
+      if constexpr (SPIN == 1)
+        se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
+      else
+        se2_point<C, T>(pt, ph, dt, ph.dim == 1 ? (T)0.5 : (T)1, sizeof(T) == 4 ? (T)1e-8 : (T)1e-17);
+      sp_point<SPIN, C, T>(pt, tb.vx[x] + vyz, ph, dt);
+      const double wx = tb.wx[x];
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        v[m][q] = pt[m];
+        acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MS; ++m) acc[m] = valid ? acc[m] * wyz : 0.0;
+    block_sum_store<MS>(acc, red, partials);
+  }
+  if (flags & XF_FWD) {
+#pragma unroll
+    for (int m = 0; m < MS; ++m) line_fft<N, Q, -1>(v[m], my, lds_tw, j);
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < MS; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) phi[m * g.npts + base + j + P * q] = v[m][q];
+  }
+}
+
+// ------------------------------------------------------- pass B/D (y)
+// Lines along y for one component; a block owns TX consecutive x at one z.
+// dir -1 forward, +1 inverse; scale applies the pending renormalisation.
+template <typename T, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale) {
+  using C = C_t<T>;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  C* lds = lds_tw + TWS;
+  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int x = blockIdx.x * TX + l;
+  const int z = blockIdx.y, m = blockIdx.z;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twy[i];
+  C* base = phi + m * g.npts + (long)z * g.nx * g.ny + x;
+  C v[Q];
+  C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    C a = base[(long)(j + P * q) * g.nx];
+    v[q] = scale ? cmul(a, sc) : a;
+  }
+  __syncthreads();
+  if (dir < 0)
+    line_fft<N, Q, -1>(v, lds + l * LS, lds_tw, j);
+  else
+    line_fft<N, Q, +1>(v, lds + l * LS, lds_tw, j);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) base[(long)(j + P * q) * g.nx] = v[q];
+}
+
+// ---------------------------------------------- pass C (k-space lines)
+// Lines along the last axis (z in 3D, y in 2D), all components: forward FFT,
+// KE (SRC/cgpe3d.f90:875-899), SOC (904-1004), inverse FFT.  A block owns TX
+// consecutive x (and one y in 3D).
+template <typename T, int SPIN, int DIM, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  C* lds_ke = lds_tw + TWS;
+  C* lds = lds_ke + MS * N;
+  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int x = blockIdx.x * TX + l;
+  const int o = DIM == 3 ? blockIdx.y : 0;  // y index in 3D
+  const long lstride = DIM == 3 ? (long)g.nx * g.ny : (long)g.nx;
+  const C* tw = DIM == 3 ? tb.twz : tb.twy;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tw[i];
+  for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
+  C* base = phi + (long)o * g.nx + x;
+  C v[MS][Q];
+#pragma unroll
+  for (int m = 0; m < MS; ++m) {
+    C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      C a = base[m * g.npts + (long)(j + P * q) * lstride];
+      v[m][q] = scale ? cmul(a, sc) : a;
+    }
+  }
+  __syncthreads();
+  C* my = lds + l * LS;
+#pragma unroll
+  for (int m = 0; m < MS; ++m) line_fft<N, Q, -1>(v[m], my, lds_tw, j);
+  const T dt = (T)ph.dt;
+  const C kexy = DIM == 3 ? cmul(tb.kea[x], tb.keb[o]) : tb.kea[x];
+  C e1, d[MS];
+  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, d);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int pos = j + P * q;
+    if constexpr (DIM == 2) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[pos], ph, dt, e1, d);
+    C pt[MS];
+#pragma unroll
+    for (int m = 0; m < MS; ++m) pt[m] = cmul(v[m][q], cmul(kexy, lds_ke[m * N + pos]));
+    soc_apply<SPIN, C, T>(pt, e1, d);
+#pragma unroll
+    for (int m = 0; m < MS; ++m) v[m][q] = pt[m];
+  }
+#pragma unroll
+  for (int m = 0; m < MS; ++m) line_fft<N, Q, +1>(v[m], my, lds_tw, j);
+#pragma unroll
+  for (int m = 0; m < MS; ++m)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) base[m * g.npts + (long)(j + P * q) * lstride] = v[m][q];
+}
+
+// ------------------------------------------------ renormalisation logic
+struct FinArgs {
+  int imag, soc, dim, ms;
+  double mag, dt;
+};
+
+// NEWTON/USRFUN (SRC/cgpe3d.f90:1516-1588) on the device
+__device__ inline int newton_dev(double* x, const double* n, double mag, double dt) {
+  double p0 = 0, p1 = 0;
+  for (int k = 0; k < 1000; ++k) {
+    double e1 = exp(2.0 * (x[0] + 2.0 * x[1]) * dt), e2 = exp(2.0 * (x[0] + x[1]) * dt), e3 = exp(2.0 * x[0] * dt),
+           e4 = exp(2.0 * (x[0] - x[1]) * dt), e5 = exp(2.0 * (x[0] - 2.0 * x[1]) * dt);
+    double f0 = e1 * n[0] + e2 * n[1] + e3 * n[2] + e4 * n[3] + e5 * n[4] - 1.0;
+    double f1 = 2.0 * e1 * n[0] + e2 * n[1] - e4 * n[3] - 2.0 * e5 * n[4] - mag;
+    double j11 = (e1 * n[0] + e2 * n[1] + e3 * n[2] + e4 * n[3] + e5 * n[4]) * 2.0 * dt;
+    double j12 = e1 * n[0] * 4.0 * dt + e2 * n[1] * 2.0 * dt + e4 * n[3] * (-2.0 * dt) + e5 * n[4] * (-4.0 * dt);
+    double j21 = (2.0 * e1 * n[0] + e2 * n[1] - e4 * n[3] - 2.0 * e5 * n[4]) * 2.0 * dt;
+    double j22 = 2.0 * e1 * n[0] * 4.0 * dt + e2 * n[1] * 2.0 * dt - e4 * n[3] * (-2.0 * dt) -
+                 2.0 * e5 * n[4] * (-4.0 * dt);
+    p1 = (f1 * j11 - f0 * j21) / (j21 * j12 - j11 * j22);
+    p0 = (-f0 - j12 * p1) / j11;
+    x[0] += p0;
+    x[1] += p1;
+    if (fmax(fabs(p0), fabs(p1)) <= 1.0e-6) break;
+  }
+  return fmax(fabs(p0), fabs(p1)) > 1.0e-6;
+}
+
+// norms n[ms] -> per-component scale c[ms] applied after the step:
+// real time 1; imaginary time sigma_m / TNORM with sigma from the SOC = 0
+// projection (SRC/cgpe3d.f90:460-491) or 1 with SO coupling (493).
+__device__ inline void renorm_scale(const FinArgs& f, const double* n, double* sig, int* err, double* c) {
+  const double EPS = 1.0e-40;
+  if (!f.imag) {
+    for (int m = 0; m < f.ms; ++m) c[m] = 1.0;
+    return;
+  }
+  double sigma[5] = {1, 1, 1, 1, 1};
+  if (!f.soc) {
+    const double mag = f.mag;
+    if (f.ms == 3) {
+      double root = sqrt(4.0 * (1.0 - mag * mag) * n[0] * n[2] + mag * mag * n[1] * n[1]);
+      sigma[1] = f.dim == 3 ? sqrt(1.0 - mag * mag) / sqrt(n[1] + root + EPS)
+                            : sqrt(1.0 - mag * mag) / (sqrt(n[1] + root) + EPS);
+      sigma[0] = sqrt(1.0 + mag - sigma[1] * sigma[1] * n[1]) / (sqrt(2.0 * n[0]) + EPS);
+      sigma[2] = sqrt(1.0 - mag - sigma[1] * sigma[1] * n[1]) / (sqrt(2.0 * n[2]) + EPS);
+    } else {
+      if (newton_dev(sig, n, mag, f.dt)) *err = 1;
+      const double dt = f.dt;
+      sigma[0] = exp((sig[0] + 2.0 * sig[1]) * dt);
+      sigma[1] = exp((sig[0] + sig[1]) * dt);
+      sigma[2] = exp(sig[0] * dt);
+      sigma[3] = exp((sig[0] - sig[1]) * dt);
+      sigma[4] = exp((sig[0] - 2.0 * sig[1]) * dt);
+    }
+  }
+  double t2 = 0;
+  for (int m = 0; m < f.ms; ++m) t2 += sigma[m] * sigma[m] * n[m];
+  const double tn = sqrt(t2);
+  for (int m = 0; m < f.ms; ++m) c[m] = sigma[m] / tn;
+}
+
+// Sum the NORMC partials in a fixed order, store the norms and the scale.
+// mode 0: step renormalisation; mode 1: plain NORMT (scale 1/TNORM).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, int nparts, FinArgs f,
+                                                   int mode, double* norm_out, C_t<T>* cscale, double* cscale64,
+                                                   double* sig, int* err) {
+  using C = C_t<T>;
+  __shared__ double red[16 * 5];
+  double a[5] = {0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    for (int m = 0; m < f.ms; ++m) a[m] += partials[(long)i * f.ms + m];
+  wave_sum<5>(a);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0)
+    for (int m = 0; m < 5; ++m) red[wid * 5 + m] = a[m];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double n[5] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < nw; ++w)
+      for (int m = 0; m < f.ms; ++m) n[m] += red[w * 5 + m];
+    double c[5];
+    if (mode == 1) {
+      double t = 0;
+      for (int m = 0; m < f.ms; ++m) t += n[m];
+      t = sqrt(t);
+      for (int m = 0; m < f.ms; ++m) c[m] = 1.0 / t;
+      norm_out[5] = t;
+    } else {
+      renorm_scale(f, n, sig, err, c);
+    }
+    for (int m = 0; m < f.ms; ++m) {
+      norm_out[m] = n[m];
+      cscale64[m] = c[m];
+      cscale[m] = mk<C>((T)c[m], (T)0);
+    }
+  }
+}
+
+// ------------------------------------------------------------------- 1D
+// One workgroup advances the whole q1D problem nsteps times: forward FFT,
+// KE, SOC with the constant basis, inverse FFT, SE, SP, NORMC, projection /
+// NORMT (SRC/cgpe1d.f90:433-474).  N = NX points, P = N/Q threads.
+template <typename T, int SPIN, int N, int Q>
+__global__ __launch_bounds__(N / Q) void k_line1d(C_t<T>* __restrict__ phi, Tables<T> tb, Phys ph, FinArgs f,
+                                                 long nsteps, double* norm_out, double* sig, int* err) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int P = N / Q;
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  double* red = reinterpret_cast<double*>(lds_tw + TWS);
+  double* bc = red + 16 * MS;  // broadcast slots
+  C* lds = reinterpret_cast<C*>(bc + 8);
+  const int j = threadIdx.x;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
+  C v[MS][Q];
+#pragma unroll
+  for (int m = 0; m < MS; ++m)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[m][q] = phi[m * N + j + P * q];
+  __syncthreads();
+  const T dt = (T)ph.dt;
+  const T inv = (T)ph.inv_npts;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  for (long it = 0; it < nsteps; ++it) {
+#pragma unroll
+    for (int m = 0; m < MS; ++m) line_fft<N, Q, -1>(v[m], lds, lds_tw, j);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int x = j + P * q;
+      C e1, d[MS];
+      soc_factors<SPIN, C, T>(tb.kx[x], (T)0, ph, dt, e1, d);
+      C pt[MS];
+      const C ke = tb.kea[x];
+#pragma unroll
+      for (int m = 0; m < MS; ++m) pt[m] = cmul(v[m][q], ke);
+      soc_apply<SPIN, C, T>(pt, e1, d);
+#pragma unroll
+      for (int m = 0; m < MS; ++m) v[m][q] = pt[m];
+    }
+#pragma unroll
+    for (int m = 0; m < MS; ++m) line_fft<N, Q, +1>(v[m], lds, lds_tw, j);
+    double acc[MS];
+#pragma unroll
+    for (int m = 0; m < MS; ++m) acc[m] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int x = j + P * q;
+      C pt[MS];
+#pragma unroll
+      for (int m = 0; m < MS; ++m) pt[m] = cscale(v[m][q], inv);
+      if constexpr (SPIN == 1)
+        se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
+      else
+        se2_point<C, T>(pt, ph, dt, (T)0.5, sizeof(T) == 4 ? (T)1e-8 : (T)1e-17);
+      sp_point<SPIN, C, T>(pt, tb.vx[x], ph, dt);
+      const double wx = tb.wx[x];
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        v[m][q] = pt[m];
+        acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
+      }
+    }
+    wave_sum<MS>(acc);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+      for (int m = 0; m < MS; ++m) red[wid * MS + m] = acc[m];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double n[5] = {0, 0, 0, 0, 0}, c[5];
+      for (int w = 0; w < nw; ++w)
+        for (int m = 0; m < MS; ++m) n[m] += red[w * MS + m];
+      renorm_scale(f, n, sig, err, c);
+      for (int m = 0; m < MS; ++m) {
+        bc[m] = c[m];
+        norm_out[m] = n[m];
+      }
+    }
+    __syncthreads();
+    if (f.imag) {
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const T c = (T)bc[m];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[m][q] = cscale(v[m][q], c);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MS; ++m)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) phi[m * N + j + P * q] = v[m][q];
+}
+
+}  // namespace fgpe

@@ -1,0 +1,323 @@
+// fgpe_ops.hpp -- templated launch code of libfgpe: pass launches with their
+// LDS budgets, runtime size dispatch, the per-run pass schedule and the
+// event-time reductions.  Instantiated per precision and spin in
+// fgpe_ops_*.hip so the kernels compile in parallel.
+#pragma once
+#include "fgpe_impl.hpp"
+
+namespace fgpe_impl {
+
+template <typename K> void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// ------------------------------------------------------------- launches
+template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
+  constexpr int Q = Cfg<T>::Q, P = N / Q;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  const int threads = P >= 256 ? P : 256;
+  const int lpw = threads / P;
+  const long nlines = (long)h->ny * h->nz;
+  const int blocks = (int)((nlines + lpw - 1) / lpw);
+  const size_t lds = TWS * sizeof(C<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
+  if (flags & XF_PHYS) {
+    int rc = ensure_partials(h, (size_t)blocks * MS);
+    if (rc) return rc;
+  }
+  auto kern = k_xpass<T, SPIN, N, Q>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
+                     phys(h), flags, h->partials);
+  HIPCHK(hipGetLastError());
+  if (flags & XF_PHYS) {
+    hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(1024), 0, h->stream, h->partials, blocks, finargs(h), 0,
+                       h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
+  constexpr int Q = Cfg<T>::Q, P = N / Q, TX = Cfg<T>::TX;
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  const size_t lds = TWS * sizeof(C<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "y pass needs %zu B of LDS", lds);
+  auto kern = k_ypass<T, N, Q, TX>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3(h->nx / TX, h->nz, h->ms), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h),
+                     tables<T>(h), dir, scale);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int SPIN, int DIM, int N> int launch_k(fgpe_handle* h, int scale) {
+  constexpr int Q = Cfg<T>::Q, P = N / Q, TX = Cfg<T>::TX;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  const size_t lds = TWS * sizeof(C<T>) + (size_t)MS * N * sizeof(C<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
+  auto kern = k_kpass<T, SPIN, DIM, N, Q, TX>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3(h->nx / TX, DIM == 3 ? h->ny : 1), dim3(TX * P), lds, h->stream, (C<T>*)h->phi,
+                     geom(h), tables<T>(h), phys(h), scale);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps) {
+  constexpr int Q = Cfg<T>::Q, P = N / Q;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  const size_t lds = TWS * sizeof(C<T>) + 16 * MS * sizeof(double) + 8 * sizeof(double) + line_stride(N) * sizeof(C<T>);
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "1D step needs %zu B of LDS", lds);
+  auto kern = k_line1d<T, SPIN, N, Q>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3(1), dim3(P), lds, h->stream, (C<T>*)h->phi, tables<T>(h), phys(h), finargs(h),
+                     nsteps, h->dnorm, h->dsig, h->derr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// runtime size dispatch
+template <typename T, int SPIN> int dispatch_x(fgpe_handle* h, int flags) {
+  switch (h->nx) {
+    case 32: return launch_x<T, SPIN, 32>(h, flags);
+    case 64: return launch_x<T, SPIN, 64>(h, flags);
+    case 128: return launch_x<T, SPIN, 128>(h, flags);
+    case 256: return launch_x<T, SPIN, 256>(h, flags);
+    case 512: return launch_x<T, SPIN, 512>(h, flags);
+  }
+  return fail(FGPE_EINVAL, "unsupported NX=%d", h->nx);
+}
+template <typename T> int dispatch_y(fgpe_handle* h, int dir, int scale) {
+  switch (h->ny) {
+    case 32: return launch_y<T, 32>(h, dir, scale);
+    case 64: return launch_y<T, 64>(h, dir, scale);
+    case 128: return launch_y<T, 128>(h, dir, scale);
+    case 256: return launch_y<T, 256>(h, dir, scale);
+    case 512: return launch_y<T, 512>(h, dir, scale);
+  }
+  return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
+}
+template <typename T, int SPIN, int DIM> int dispatch_k(fgpe_handle* h, int scale) {
+  const int n = DIM == 3 ? h->nz : h->ny;
+  switch (n) {
+    case 32: return launch_k<T, SPIN, DIM, 32>(h, scale);
+    case 64: return launch_k<T, SPIN, DIM, 64>(h, scale);
+    case 128: return launch_k<T, SPIN, DIM, 128>(h, scale);
+    case 256: return launch_k<T, SPIN, DIM, 256>(h, scale);
+    case 512: return launch_k<T, SPIN, DIM, 512>(h, scale);
+  }
+  return fail(FGPE_EINVAL, "unsupported line length %d", n);
+}
+template <typename T, int SPIN> int dispatch_1d(fgpe_handle* h, long nsteps) {
+  switch (h->nx) {
+    case 64: return launch_1d<T, SPIN, 64>(h, nsteps);
+    case 128: return launch_1d<T, SPIN, 128>(h, nsteps);
+    case 256: return launch_1d<T, SPIN, 256>(h, nsteps);
+    case 512: return launch_1d<T, SPIN, 512>(h, nsteps);
+    case 1024: return launch_1d<T, SPIN, 1024>(h, nsteps);
+    case 2048: return launch_1d<T, SPIN, 2048>(h, nsteps);
+    case 4096: return launch_1d<T, SPIN, 4096>(h, nsteps);
+  }
+  return fail(FGPE_EINVAL, "unsupported 1D NX=%d", h->nx);
+}
+
+template <typename T, int SPIN> int run_steps(fgpe_handle* h, long nsteps) {
+  if (h->dim == 1) {
+    long done = 0;
+    while (done < nsteps) {
+      long n = std::min<long>(nsteps - done, 4096);
+      int rc = dispatch_1d<T, SPIN>(h, n);
+      if (rc) return rc;
+      done += n;
+    }
+    return 0;
+  }
+  int rc;
+  hipLaunchKernelGGL(k_set_scale_one<T>, dim3(1), dim3(64), 0, h->stream, (C<T>*)h->cscale, h->ms);
+  if ((rc = dispatch_x<T, SPIN>(h, XF_FWD))) return rc;
+  for (long s = 1; s <= nsteps; ++s) {
+    if (h->dim == 3) {
+      if ((rc = dispatch_y<T>(h, -1, 1))) return rc;
+      if ((rc = dispatch_k<T, SPIN, 3>(h, 0))) return rc;
+      if ((rc = dispatch_y<T>(h, +1, 0))) return rc;
+    } else {
+      if ((rc = dispatch_k<T, SPIN, 2>(h, 1))) return rc;
+    }
+    const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0);
+    if ((rc = dispatch_x<T, SPIN>(h, flags))) return rc;
+  }
+  if (h->p.imag) {
+    hipLaunchKernelGGL(k_apply_scale<T>, dim3(grid_for(h->npts * h->ms, 256)), dim3(256), 0, h->stream,
+                       (C<T>*)h->phi, h->npts, h->ms, (const C<T>*)h->cscale);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+template <typename T> int normc_t(fgpe_handle* h, double* norm) {
+  const int blocks = grid_for(h->npts, 256) > 2048 ? 2048 : grid_for(h->npts, 256);
+  int rc = ensure_partials(h, (size_t)blocks * h->ms);
+  if (rc) return rc;
+  if (h->ms == 3)
+    hipLaunchKernelGGL((k_norm_reduce<T, 3>), dim3(blocks), dim3(256), 0, h->stream, (const C<T>*)h->phi, geom(h),
+                       h->wx, h->wy, h->wz, h->partials);
+  else
+    hipLaunchKernelGGL((k_norm_reduce<T, 5>), dim3(blocks), dim3(256), 0, h->stream, (const C<T>*)h->phi, geom(h),
+                       h->wx, h->wy, h->wz, h->partials);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part((size_t)blocks * h->ms);
+  HIPCHK(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int m = 0; m < h->ms; ++m) {
+    double s = 0;
+    for (int b = 0; b < blocks; ++b) s += part[(size_t)b * h->ms + m];
+    norm[m] = s;
+  }
+  return 0;
+}
+
+template <typename T> int normalize_t(fgpe_handle* h, double* tnorm) {
+  double n[5];
+  int rc = normc_t<T>(h, n);
+  if (rc) return rc;
+  double t = 0;
+  for (int m = 0; m < h->ms; ++m) t += n[m];
+  t = std::sqrt(t);
+  std::vector<C<T>> c(h->ms);
+  for (int m = 0; m < h->ms; ++m) {
+    c[m].x = (T)(1.0 / t);
+    c[m].y = (T)0;
+  }
+  HIPCHK(hipMemcpyAsync(h->cscale, c.data(), c.size() * sizeof(C<T>), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_apply_scale<T>, dim3(grid_for(h->npts * h->ms, 256)), dim3(256), 0, h->stream, (C<T>*)h->phi,
+                     h->npts, h->ms, (const C<T>*)h->cscale);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (tnorm) *tnorm = t;
+  return 0;
+}
+
+template <typename T, int SPIN> int observables_t(fgpe_handle* h, double* en, double* mu, double* mz, double* rms) {
+  const int blocks = std::min(grid_for(h->npts, 256), 4096);
+  const int nc = 3 * h->ms + 2;
+  int rc = ensure_partials(h, (size_t)blocks * nc);
+  if (rc) return rc;
+  ObsArgs a;
+  a.tau0 = h->p.tau[0];
+  a.tau1 = h->p.tau[1];
+  a.tau2 = h->p.tau[2];
+  a.gx = h->p.gamma[0];
+  a.gy = h->p.gamma[1];
+  a.gz = h->p.gamma[2];
+  a.dx = h->p.d[0];
+  a.dy = h->p.d[1];
+  a.dz = h->p.d[2];
+  a.wx = h->wx;
+  a.wy = h->wy;
+  a.wz = h->wz;
+  a.x2 = h->x2;
+  a.y2 = h->y2;
+  a.z2 = h->z2;
+  a.vxd = h->vxd;
+  a.vyd = h->vyd;
+  a.vzd = h->vzd;
+  const C<T>* ph = (const C<T>*)h->phi;
+  Geom g = geom(h);
+#define OBS(D) hipLaunchKernelGGL((k_observables<T, SPIN, D>), dim3(blocks), dim3(256), 0, h->stream, ph, g, a, h->partials)
+  if (h->dim == 1)
+    OBS(1);
+  else if (h->dim == 2)
+    OBS(2);
+  else
+    OBS(3);
+#undef OBS
+  HIPCHK(hipGetLastError());
+  std::vector<double> part((size_t)blocks * nc);
+  HIPCHK(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> s(nc, 0.0);
+  for (int b = 0; b < blocks; ++b)
+    for (int c = 0; c < nc; ++c) s[c] += part[(size_t)b * nc + c];
+  const int ms = h->ms;
+  double sumn = 0;
+  for (int m = 0; m < ms; ++m) sumn += s[2 * ms + 2 + m];
+  // EN = int col1 / sum(NORM), MZ = int col2, MU = int col/(NORM+EPS),
+  // RMS = sqrt(int r^2 |phi|^2) (SRC/cgpe3d.f90:1506-1510,1265-1267)
+  if (en) *en = s[0] / sumn;
+  if (mz) *mz = s[1];
+  for (int m = 0; m < ms; ++m) {
+    if (mu) mu[m] = s[2 + m] / (s[2 * ms + 2 + m] + EPS);
+    if (rms) rms[m] = std::sqrt(s[ms + 2 + m]);
+  }
+  return 0;
+}
+
+template <typename T> int conv_t(fgpe_handle* h, double* out) {
+  const long n = h->npts * h->ms;
+  const int blocks = std::min(grid_for(n, 256), 4096);
+  int rc = ensure_partials(h, blocks);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_conv_max<T>, dim3(blocks), dim3(256), 0, h->stream, (const C<T>*)h->phi,
+                     (const C<T>*)h->phi_old, n, h->partials);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part(blocks);
+  HIPCHK(hipMemcpyAsync(part.data(), h->partials, blocks * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double mx = 0;
+  for (double v : part) mx = std::max(mx, v);
+  *out = mx;
+  return 0;
+}
+
+template <typename T> int guess_t(fgpe_handle* h) {
+  GuessArgs a;
+  a.dim = h->dim;
+  a.spin = h->spin;
+  a.fpc = h->p.option_fpc;
+  a.ax = h->p.alpha[0];
+  a.ay = h->p.alpha[1];
+  a.az = h->p.alpha[2];
+  a.t0 = h->p.tau[0];
+  a.t1 = h->p.tau[1];
+  a.t2 = h->p.tau[2];
+  a.mag = h->p.mag;
+  a.x2 = h->x2;
+  a.y2 = h->y2;
+  a.z2 = h->z2;
+  hipLaunchKernelGGL(k_guess<T>, dim3(grid_for(h->npts, 256)), dim3(256), 0, h->stream, (C<T>*)h->phi, geom(h), a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+
+
+template <typename T> int from64_t(fgpe_handle* h, long n) {
+  hipLaunchKernelGGL(k_from64<T>, dim3(grid_for(n, 256)), dim3(256), 0, h->stream, (const double2*)h->stage64,
+                     (C<T>*)h->phi, n);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+template <typename T> int to64_t(fgpe_handle* h, long n) {
+  hipLaunchKernelGGL(k_to64<T>, dim3(grid_for(n, 256)), dim3(256), 0, h->stream, (const C<T>*)h->phi,
+                     (double2*)h->stage64, n);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fgpe_impl
+
+#define FGPE_INSTANTIATE_SPIN(T, S)                                                                    \
+  template int fgpe_impl::run_steps<T, S>(fgpe_handle*, long);                                         \
+  template int fgpe_impl::observables_t<T, S>(fgpe_handle*, double*, double*, double*, double*);
+#define FGPE_INSTANTIATE_PREC(T)                                         \
+  template int fgpe_impl::normc_t<T>(fgpe_handle*, double*);             \
+  template int fgpe_impl::normalize_t<T>(fgpe_handle*, double*);         \
+  template int fgpe_impl::conv_t<T>(fgpe_handle*, double*);              \
+  template int fgpe_impl::guess_t<T>(fgpe_handle*);                      \
+  template int fgpe_impl::from64_t<T>(fgpe_handle*, long);               \
+  template int fgpe_impl::to64_t<T>(fgpe_handle*, long);

@@ -1,0 +1,184 @@
+"""ctypes binding of libfgpe (include/fgpe.h).
+
+``Solver`` owns one fgpe handle.  Wave functions are numpy complex128 arrays
+of shape (ms, nz, ny, nx), which is the reference's PHI(NX,NY,NZ,MS) Fortran
+layout read as C order.  There is no CPU fallback: loading fails loudly when
+the library is missing, and every compute call raises when the HIP runtime
+reports an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import FGPE_FP32, FGPE_FP64, FgpeInput, FgpeParams  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfgpe.so")
+
+_lib = None
+
+EXPORTS = (
+    "fgpe_derive",
+    "fgpe_create",
+    "fgpe_destroy",
+    "fgpe_initial_guess",
+    "fgpe_upload",
+    "fgpe_download",
+    "fgpe_normalize",
+    "fgpe_run",
+    "fgpe_last_norms",
+    "fgpe_normc",
+    "fgpe_observables",
+    "fgpe_keep_old",
+    "fgpe_conv_max",
+    "fgpe_center_abs",
+    "fgpe_sync",
+    "fgpe_last_run_ms",
+    "fgpe_last_error",
+)
+
+
+class FgpeError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FgpeError("libfgpe.so is not built (run __graft_entry__.build()): %s" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, dp, i32, i64 = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int64
+        L.fgpe_derive.argtypes = [C.POINTER(FgpeInput), i32, C.POINTER(FgpeParams), dp]
+        L.fgpe_create.argtypes = [C.POINTER(FgpeParams), C.POINTER(vp)]
+        L.fgpe_destroy.argtypes = [vp]
+        L.fgpe_destroy.restype = None
+        for name in ("fgpe_initial_guess", "fgpe_keep_old", "fgpe_sync"):
+            getattr(L, name).argtypes = [vp]
+        L.fgpe_upload.argtypes = [vp, dp]
+        L.fgpe_download.argtypes = [vp, dp]
+        L.fgpe_normalize.argtypes = [vp, dp]
+        L.fgpe_run.argtypes = [vp, i64]
+        L.fgpe_last_norms.argtypes = [vp, dp]
+        L.fgpe_normc.argtypes = [vp, dp]
+        L.fgpe_observables.argtypes = [vp, dp, dp, dp, dp]
+        L.fgpe_conv_max.argtypes = [vp, dp]
+        L.fgpe_center_abs.argtypes = [vp, dp]
+        L.fgpe_last_run_ms.argtypes = [vp, dp]
+        L.fgpe_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        for name in EXPORTS:
+            if getattr(L, name).restype is not None and name != "fgpe_destroy":
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(512)
+    lib().fgpe_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise FgpeError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def derive(inp, precision=FGPE_FP64):
+    p = FgpeParams()
+    aosc = C.c_double()
+    _check(lib().fgpe_derive(C.byref(inp), precision, C.byref(p), C.byref(aosc)), "fgpe_derive")
+    return p, aosc.value
+
+
+class Solver:
+    """One propagator instance on the current HIP device."""
+
+    def __init__(self, params):
+        self.p = params
+        self.ms = 2 * params.spin + 1
+        self.shape = (self.ms, params.n[2] if params.dim == 3 else 1, params.n[1] if params.dim >= 2 else 1,
+                      params.n[0])
+        h = C.c_void_p()
+        _check(lib().fgpe_create(C.byref(params), C.byref(h)), "fgpe_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgpe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def initial_guess(self):
+        _check(lib().fgpe_initial_guess(self.h), "fgpe_initial_guess")
+
+    def upload(self, phi):
+        a = np.ascontiguousarray(np.asarray(phi, dtype=np.complex128).reshape(self.shape))
+        _check(lib().fgpe_upload(self.h, _dp(a.view(np.float64))), "fgpe_upload")
+
+    def download(self):
+        a = np.empty(self.shape, dtype=np.complex128)
+        _check(lib().fgpe_download(self.h, _dp(a.view(np.float64))), "fgpe_download")
+        return a
+
+    def normalize(self):
+        t = C.c_double()
+        _check(lib().fgpe_normalize(self.h, C.byref(t)), "fgpe_normalize")
+        return t.value
+
+    def run(self, nsteps):
+        _check(lib().fgpe_run(self.h, int(nsteps)), "fgpe_run")
+
+    def last_norms(self):
+        a = np.empty(self.ms)
+        _check(lib().fgpe_last_norms(self.h, _dp(a)), "fgpe_last_norms")
+        return a
+
+    def normc(self):
+        a = np.empty(self.ms)
+        _check(lib().fgpe_normc(self.h, _dp(a)), "fgpe_normc")
+        return a
+
+    def observables(self):
+        """(EN, MU[ms], MZ, RMS[ms]) in the reference's scaled units."""
+        en, mz = C.c_double(), C.c_double()
+        mu, rms = np.empty(self.ms), np.empty(self.ms)
+        _check(lib().fgpe_observables(self.h, C.byref(en), _dp(mu), C.byref(mz), _dp(rms)), "fgpe_observables")
+        return en.value, mu, mz.value, rms
+
+    def keep_old(self):
+        _check(lib().fgpe_keep_old(self.h), "fgpe_keep_old")
+
+    def conv_max(self):
+        v = C.c_double()
+        _check(lib().fgpe_conv_max(self.h, C.byref(v)), "fgpe_conv_max")
+        return v.value
+
+    def center_abs(self):
+        a = np.empty(self.ms)
+        _check(lib().fgpe_center_abs(self.h, _dp(a)), "fgpe_center_abs")
+        return a
+
+    def sync(self):
+        _check(lib().fgpe_sync(self.h), "fgpe_sync")
+
+    def last_run_ms(self):
+        v = C.c_double()
+        _check(lib().fgpe_last_run_ms(self.h, C.byref(v)), "fgpe_last_run_ms")
+        return v.value

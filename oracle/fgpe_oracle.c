@@ -536,6 +536,49 @@ static void soc_a_1d(int spin, double* a) {
   }
 }
 
+/* SOC factor at one k-point, for unit checks: 1D (dim == 1) uses the
+ * constant basis and exponent 2 L GAMMAX kx (SRC/cgpe1d.f90:1516-1520),
+ * 2D/3D the A(beta) basis (SRC/cgpe3d.f90:956-998). v, out: ms interleaved
+ * complex. */
+void or_soc_point(int spin, int dim, double kx, double ky, double gx, double gy, double cdt_re, double cdt_im,
+                  const double* v, double* out) {
+  const int S = spin, ms = 2 * spin + 1;
+  cplx cdt = cdt_re + I * cdt_im;
+  cplx x[5], w[5];
+  for (int m = 0; m < ms; m++) x[m] = v[2 * m] + I * v[2 * m + 1];
+  if (dim == 1) {
+    double a[25];
+    soc_a_1d(S, a);
+    for (int r = 0; r < ms; r++) {
+      w[r] = 0;
+      for (int c = 0; c < ms; c++) w[r] += a[c * ms + r] * x[c];
+    }
+    for (int L = -S; L <= S; L++) w[L + S] = cexp(-I * cdt * (2.0 * L * gx * kx)) * w[L + S];
+    for (int r = 0; r < ms; r++) {
+      x[r] = 0;
+      for (int c = 0; c < ms; c++) x[r] += a[r * ms + c] * w[c];
+    }
+  } else {
+    double alpha = 2.0 * sqrt(gx * gx * kx * kx + gy * gy * ky * ky);
+    double beta = alpha <= EPS ? 0.0 : atan2(gy * ky, gx * kx);
+    cplx a[25];
+    soc_a_beta(S, beta, a);
+    for (int r = 0; r < ms; r++) {
+      w[r] = 0;
+      for (int c = 0; c < ms; c++) w[r] += conj(a[c * ms + r]) * x[c];
+    }
+    for (int L = -S; L <= S; L++) w[L + S] *= cexp(-I * cdt * (L * alpha));
+    for (int r = 0; r < ms; r++) {
+      x[r] = 0;
+      for (int c = 0; c < ms; c++) x[r] += a[r * ms + c] * w[c];
+    }
+  }
+  for (int m = 0; m < ms; m++) {
+    out[2 * m] = creal(x[m]);
+    out[2 * m + 1] = cimag(x[m]);
+  }
+}
+
 /* -------------------------------------------------------------------- SOC
  * 2D/3D: per (kx,ky), ALPHA = 2 sqrt(gx^2 kx^2 + gy^2 ky^2), BETA = atan2 (0
  * when ALPHA <= EPS), PHIF <- A diag(exp(-i CDT L ALPHA)) A^H PHIF

@@ -45,6 +45,8 @@ void or_ke(or_state* s);
 void or_soc(or_state* s);
 void or_se(or_state* s);
 void or_sp(or_state* s);
+void or_soc_point(int spin, int dim, double kx, double ky, double gx, double gy, double cdt_re, double cdt_im,
+                  const double* v, double* out);
 void or_expm_herm5(const double* h, double cdt_re, double cdt_im, const double* v, double* out);
 
 #ifdef __cplusplus

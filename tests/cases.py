@@ -1,0 +1,32 @@
+"""Shared small configurations for the oracle/GPU parity tests.
+
+Physical parameters follow the benchmark configurations (87Rb spin-2:
+A0/A2/A4 = 87.93/91.28/99.18 a_B; spin-1: 101.8/100.4 a_B) and the
+reference's committed 1D run (23Na, INPUT/input1D.dat), at grid sizes the
+FP64 oracle finishes in seconds.
+"""
+import numpy as np
+
+from fortress_amd.abi import make_input
+
+RB_SPIN2 = dict(mass=87.0, a=(87.93, 91.28, 99.18))
+RB_SPIN1 = dict(mass=87.0, a=(101.8, 100.4))
+NA_SPIN2 = dict(mass=23.0, a=(34.9, 45.8, 64.5))
+
+
+def case(dim, spin, n, d, dt, gamma, imag=1, natoms=10000, nu=(20.0, 400.0, 400.0), option_fpc=4, mag=0.0,
+         species=None):
+    sp = species or (RB_SPIN2 if spin == 2 else RB_SPIN1)
+    return make_input(dim, spin, n, d, dt, sp["mass"], sp["a"], natoms, nu, gamma, switch_im=imag,
+                      option_fpc=option_fpc, mag=mag)
+
+
+def noisy(phi, seed, amp=0.01):
+    """phi + amp*max|phi|*(xi_r + i xi_i)/sqrt(2), xi ~ N(0,1), fixed seed."""
+    rng = np.random.default_rng(seed)
+    scale = amp * np.abs(phi).max() / np.sqrt(2.0)
+    return phi + scale * (rng.standard_normal(phi.shape) + 1j * rng.standard_normal(phi.shape))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))

@@ -1,0 +1,106 @@
+"""GPU parity: libfgpe (HIP, gfx950) against the FP64 CPU oracle.
+
+Each case starts both from the same state (the reference's initial guess,
+renormalised in imaginary time; guess plus seeded 1% complex noise in real
+time), advances the same number of steps and compares the wave function
+(relative L2), the NORMC of the last step and the observables.  Bars:
+FP64 propagation 1e-10 relative; FP32 propagation 2e-5 relative L2 on phi and
+1e-5 relative on the FP64-reduced observables.
+"""
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+from cases import NA_SPIN2, case, noisy, rel_l2
+from fortress_amd import fgpe
+from fortress_amd.abi import FGPE_FP32, FGPE_FP64
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "1d_s2_soc_imag": (case(1, 2, (256,), (0.05,), 1.25e-4, (0.5,), species=NA_SPIN2), 40),
+    "1d_s2_soc_real": (case(1, 2, (512,), (0.05,), 1.25e-4, (0.5,), imag=0, species=NA_SPIN2), 40),
+    "1d_s1_soc_imag": (case(1, 1, (512,), (0.05,), 5e-4, (0.5,)), 40),
+    "1d_s1_soc_real": (case(1, 1, (512,), (0.05,), 5e-4, (0.5,), imag=0), 40),
+    "1d_s2_nosoc_newton": (case(1, 2, (256,), (0.05,), 1.25e-4, (0.0,), mag=0.5, species=NA_SPIN2), 40),
+    "1d_s1_nosoc_sigma": (case(1, 1, (256,), (0.05,), 5e-4, (0.0,), mag=0.3), 40),
+    "2d_s2_soc_imag": (case(2, 2, (64, 64), (0.1, 0.1), 1e-3, (1.0, 1.0), natoms=100000, nu=(20, 20, 400)), 20),
+    "2d_s2_soc_real": (case(2, 2, (64, 32), (0.2, 0.2), 1e-3, (1.0, 0.5), imag=0, natoms=100000,
+                            nu=(20, 20, 400)), 20),
+    "2d_s1_soc_imag": (case(2, 1, (64, 64), (0.1, 0.1), 1e-3, (0.7, 1.3), natoms=50000, nu=(20, 25, 400)), 20),
+    "2d_s1_soc_real": (case(2, 1, (32, 64), (0.2, 0.1), 1e-3, (0.7, 1.3), imag=0, natoms=50000,
+                            nu=(20, 25, 400)), 20),
+    "2d_s1_nosoc_sigma": (case(2, 1, (64, 64), (0.1, 0.1), 1e-3, (0.0, 0.0), mag=0.2, nu=(20, 20, 400)), 20),
+    "3d_s2_soc_imag": (case(3, 2, (32, 32, 32), (0.4, 0.4, 0.4), 5e-4, (0.5, 0.5, 0.5), natoms=100000,
+                            nu=(20, 20, 20)), 10),
+    "3d_s2_soc_real": (case(3, 2, (32, 64, 32), (0.4, 0.2, 0.4), 5e-4, (0.5, 0.3, 0.7), imag=0, natoms=100000,
+                            nu=(20, 20, 20)), 10),
+    "3d_s1_soc_imag": (case(3, 1, (32, 32, 64), (0.4, 0.4, 0.2), 5e-4, (0.5, 0.5, 0.5), natoms=100000,
+                            nu=(20, 20, 20)), 10),
+    "3d_s1_soc_real": (case(3, 1, (64, 32, 32), (0.2, 0.4, 0.4), 5e-4, (0.3, 0.6, 0.9), imag=0, natoms=100000,
+                            nu=(20, 20, 20)), 10),
+    "3d_s2_nosoc_newton": (case(3, 2, (32, 32, 32), (0.4, 0.4, 0.4), 5e-4, (0.0, 0.0, 0.0), mag=0.4,
+                                natoms=100000, nu=(20, 20, 20)), 10),
+}
+
+
+def start_state(params, seed=1):
+    o = O.Oracle(params)
+    o.initial_guess()
+    o.normt()
+    phi = o.get_phi()
+    if not params.imag:
+        phi = noisy(phi, seed)
+    o.close()
+    return phi
+
+
+def run_pair(name, precision):
+    inp, nsteps = CASES[name]
+    p_or, _ = O.derive(inp)
+    p_gpu, _ = fgpe.derive(inp, precision)
+    phi0 = start_state(p_or)
+    o = O.Oracle(p_or)
+    o.set_phi(phi0)
+    o.run(nsteps)
+    ref = o.get_phi()
+    with fgpe.Solver(p_gpu) as s:
+        s.upload(phi0)
+        s.run(nsteps)
+        got = s.download()
+        gn = s.last_norms()
+        gobs = s.observables()
+    rn = o.last_norms()
+    robs = o.energy(), o.rad()
+    o.close()
+    return ref, got, rn, gn, robs, gobs
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_parity(name):
+    print("case", name, "fp64", flush=True)
+    ref, got, rn, gn, (en_mu_mz, rms), (en, mu, mz, grms) = run_pair(name, FGPE_FP64)
+    err = rel_l2(got, ref)
+    print("  rel_l2 phi", err, flush=True)
+    assert err < 1e-10
+    assert np.allclose(gn, rn, rtol=1e-10, atol=1e-14)
+    ren, rmu, rmz = en_mu_mz
+    assert abs(en - ren) <= 1e-10 * abs(ren)
+    assert np.allclose(mu, rmu, rtol=1e-10, atol=1e-12)
+    assert abs(mz - rmz) <= 1e-10 * max(1.0, abs(rmz))
+    assert np.allclose(grms, rms, rtol=1e-10)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp32_parity(name):
+    print("case", name, "fp32", flush=True)
+    ref, got, rn, gn, (en_mu_mz, rms), (en, mu, mz, grms) = run_pair(name, FGPE_FP32)
+    err = rel_l2(got, ref)
+    print("  rel_l2 phi", err, flush=True)
+    assert err < 2e-5
+    assert np.allclose(gn, rn, rtol=1e-5, atol=1e-9)
+    ren, rmu, rmz = en_mu_mz
+    assert abs(en - ren) <= 1e-5 * abs(ren)
+    assert np.allclose(mu, rmu, rtol=1e-5, atol=1e-7)
+    assert abs(mz - rmz) <= 1e-5 * max(1.0, abs(rmz))
+    assert np.allclose(grms, rms, rtol=1e-5)

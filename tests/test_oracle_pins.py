@@ -1,0 +1,460 @@
+"""Pins of the FP64 CPU oracle (oracle/fgpe_oracle.c).
+
+The oracle is only trusted as far as these checks reach:
+  * the reference's committed run (OUTPUT/ of a 1D spin-2 SOC
+    imaginary-time run with INPUT/input1D.dat, copied to tests/golden/):
+    header constants, the t=0 row, the first STP rows of file2/file3 and the
+    first convergence.dat row (CONV/CONV_EN semantics);
+  * closed forms: Simpson exact on cubics, DIFF exact on polynomials inside and
+    on periodic functions at the wrapped ends, the DFT against numpy, the SOC
+    factor against expm(-i CDT 2 (g.k).S) with the standard spin matrices, the
+    spin-1 SE against a 3x3 expm and the spin-2 exponential against a 5x5 expm;
+  * consistency with the mathematics: the SE+SP generator equals the
+    derivative of the interaction energy that ENERGY integrates (all spins,
+    all dimensions), the predictor factor of the spin-2 SE step (1/2 in 1D,
+    1 in 2D/3D) from the local error order against a fine ODE solve, and
+    ENERGY/MU for 2D/3D against a spectral evaluation of <phi|H|phi> with the
+    propagator's own Hamiltonian.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+from scipy.integrate import solve_ivp
+
+import oracle_ctypes as O
+from cases import NA_SPIN2, case
+from fortress_amd.inputs import read_input
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+S5 = np.sqrt(0.5)
+R15 = np.sqrt(1.5)
+SPIN_MATS = {
+    1: (S5 * np.array([[0, 1, 0], [1, 0, 1], [0, 1, 0]], complex),
+        S5 * np.array([[0, -1j, 0], [1j, 0, -1j], [0, 1j, 0]]),
+        np.diag([1.0, 0.0, -1.0]).astype(complex)),
+    2: (np.array([[0, 1, 0, 0, 0], [1, 0, R15, 0, 0], [0, R15, 0, R15, 0], [0, 0, R15, 0, 1], [0, 0, 0, 1, 0]],
+                 complex),
+        np.array([[0, -1j, 0, 0, 0], [1j, 0, -1j * R15, 0, 0], [0, 1j * R15, 0, -1j * R15, 0],
+                  [0, 0, 1j * R15, 0, -1j], [0, 0, 0, 1j, 0]]),
+        np.diag([2.0, 1.0, 0.0, -1.0, -2.0]).astype(complex)),
+}
+
+
+def golden_oracle():
+    inp = read_input(os.path.join(GOLD, "input1D.dat"), 1)
+    p, aosc = O.derive(inp)
+    return inp, p, aosc
+
+
+# ------------------------------------------------------------------ golden
+def test_golden_header_constants():
+    inp, p, aosc = golden_oracle()
+    lines = open(os.path.join(GOLD, "file1_im_spin2.dat")).read().splitlines()
+    assert "AOSC =   0.4688E-05" in lines[11]
+    want = [float(t) for t in lines[13].replace(",", " ").split() if t[0] in "-0123456789"]
+    assert np.allclose([p.tau[0] / 2, p.tau[1] / 2, p.tau[2] / 2], want, atol=5e-7)
+    assert "%.4E" % aosc == "4.6883E-06"
+    assert p.soc == 1 and p.option_fpc == 4 and p.imag == 1
+
+
+def test_golden_initial_row():
+    inp, p, _ = golden_oracle()
+    o = O.Oracle(p)
+    o.initial_guess()
+    o.normt()
+    norm = o.normc()
+    en, mu, mz = o.energy()
+    rms = o.rad()
+    f2 = np.loadtxt(os.path.join(GOLD, "file2_im_spin2.dat"), max_rows=1)
+    f3 = np.loadtxt(os.path.join(GOLD, "file3_im_spin2.dat"), max_rows=1)
+    assert abs(en / 2 - f2[1]) < 5e-10
+    assert np.allclose(rms, f2[2:], atol=5e-11)
+    assert np.allclose(norm, f3[1:6], atol=5e-11)
+    row = open(os.path.join(GOLD, "file1_im_spin2.dat")).read().splitlines()[18].split()
+    # 'Initial :' NORM EN MU1..MU5 phi1..phi5
+    vals = [float(t) for t in row[2:]]
+    assert abs(sum(norm) - vals[0]) < 5e-5
+    assert np.allclose(mu / 2, vals[2:7], atol=5e-6)
+    o.close()
+
+
+def test_golden_first_rows_and_convergence():
+    """Three STP bins of the committed run and the first convergence row."""
+    inp, p, _ = golden_oracle()
+    o = O.Oracle(p)
+    o.initial_guess()
+    o.normt()
+    en0 = o.energy()[0]
+    f2 = np.loadtxt(os.path.join(GOLD, "file2_im_spin2.dat"), max_rows=4)
+    f3 = np.loadtxt(os.path.join(GOLD, "file3_im_spin2.dat"), max_rows=4)
+    cv = np.loadtxt(os.path.join(GOLD, "convergence.dat"), max_rows=1)
+    stp, dt = inp.stp, inp.dt
+    o.run(stp - 1)
+    phi_old = o.get_phi()
+    o.run(1)
+    en, mu, mz = o.energy()
+    conv = np.abs(o.get_phi() - phi_old).max() / (stp * 2 * dt)
+    conv_en = abs(en - en0) / (stp * 2 * dt)
+    assert abs(conv - cv[1]) <= 1e-7 * cv[1]
+    assert abs(conv_en - cv[2]) <= 1e-7 * cv[2]
+    for r in range(1, 4):
+        if r > 1:
+            o.run(stp)
+        en, mu, mz = o.energy()
+        rms = o.rad()
+        n = o.last_norms()
+        assert abs(2 * stp * r * dt - f2[r, 0]) < 1e-9
+        assert abs(en / 2 - f2[r, 1]) < 2e-10
+        assert np.allclose(rms, f2[r, 2:], atol=1e-10)
+        assert np.allclose(n, f3[r, 1:6], atol=1e-10)
+        assert abs(n.sum() - f3[r, 6]) < 1e-10
+        assert abs(mz - f3[r, 7]) < 1e-10
+    o.close()
+
+
+# ------------------------------------------------------------ closed forms
+@pytest.mark.parametrize("n", [16, 64, 192, 256, 2048])
+def test_simpson_exact_on_cubics(n):
+    dx = 0.37
+    x = np.arange(n) * dx
+    rng = np.random.default_rng(n)
+    c = rng.standard_normal(4)
+    f = c[0] + c[1] * x + c[2] * x**2 + c[3] * x**3
+    L = x[-1]
+    exact = c[0] * L + c[1] * L**2 / 2 + c[2] * L**3 / 3 + c[3] * L**4 / 4
+    assert abs(O.simpson(f, dx) - exact) <= 1e-12 * max(1.0, abs(exact))
+    assert abs(O.simpson(np.ones(n), dx) - (n - 1) * dx) < 1e-12 * n
+
+
+def test_simpson_is_not_trapezoid():
+    # a quartic is not integrated exactly: guards against a rule of the wrong order
+    n, dx = 64, 0.1
+    x = np.arange(n) * dx
+    assert abs(O.simpson(x**4, dx) - x[-1] ** 5 / 5) > 1e-9
+
+
+def test_diff_exact_inside_on_polynomials():
+    n, dx = 64, 0.05
+    x = np.arange(n) * dx
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(9)
+    p = sum(c[k] * x**k for k in range(9))
+    dp = sum(k * c[k] * x ** (k - 1) for k in range(1, 9))
+    got = O.diff(p, dx)
+    inner = slice(4, n - 4)
+    assert np.allclose(got[inner], dp[inner], rtol=1e-9, atol=1e-9)
+    # five-point rows are exact for degree <= 4
+    q = c[0] + c[1] * x + c[2] * x**2 + c[3] * x**3 + c[4] * x**4
+    dq = c[1] + 2 * c[2] * x + 3 * c[3] * x**2 + 4 * c[4] * x**3
+    g = O.diff(q, dx)
+    for i in (2, 3, n - 4, n - 3):
+        assert abs(g[i] - dq[i]) < 1e-9 * max(1, abs(dq[i]))
+
+
+def test_diff_wraps_periodically_at_the_ends():
+    n = 128
+    L = 6.4
+    dx = L / n
+    x = np.arange(n) * dx
+    f = np.sin(2 * np.pi * x / L) + 0.3 * np.cos(4 * np.pi * x / L)
+    df = (2 * np.pi / L) * np.cos(2 * np.pi * x / L) - 0.3 * (4 * np.pi / L) * np.sin(4 * np.pi * x / L)
+    got = O.diff(f, dx)
+    for i in (0, 1, n - 2, n - 1):
+        assert abs(got[i] - df[i]) < 2e-5
+    assert np.abs(got[4:-4] - df[4:-4]).max() < 1e-9
+
+
+@pytest.mark.parametrize("n", [2, 3, 6, 12, 64, 96, 192, 256, 2048])
+def test_dft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.allclose(O.dft(z, -1), np.fft.fft(z), rtol=0, atol=1e-12 * n)
+    assert np.allclose(O.dft(z, +1), np.fft.ifft(z) * n, rtol=0, atol=1e-12 * n)
+
+
+def _soc_point(spin, dim, kx, ky, gx, gy, cdt, v):
+    L = O.lib()
+    L.or_soc_point.argtypes = [C.c_int, C.c_int] + [C.c_double] * 6 + [C.POINTER(C.c_double)] * 2
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    out = np.empty_like(v)
+    L.or_soc_point(spin, dim, kx, ky, gx, gy, cdt.real, cdt.imag, O._dp(v.view(np.float64)),
+                   O._dp(out.view(np.float64)))
+    return out
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("imag", [0, 1])
+def test_soc_factor_is_expm_of_spin_orbit_generator(spin, dim, imag):
+    sx, sy, _ = SPIN_MATS[spin]
+    dt = 2e-3
+    cdt = -1j * dt if imag else dt + 0j
+    gx, gy = 0.5, (0.8 if dim == 2 else 0.0)
+    rng = np.random.default_rng(spin * 10 + dim)
+    ms = 2 * spin + 1
+    for kx, ky in ((0.0, 0.0), (3.0, 0.0), (0.0, -2.0), (1.3, -2.2), (-4.0, 1.0)):
+        if dim == 1:
+            ky = 0.0
+        v = rng.standard_normal(ms) + 1j * rng.standard_normal(ms)
+        g = 2 * (gx * kx * sx + gy * ky * sy)
+        want = sl.expm(-1j * cdt * g) @ v
+        assert np.allclose(_soc_point(spin, dim, kx, ky, gx, gy, cdt, v), want, atol=1e-13)
+
+
+@pytest.mark.parametrize("imag", [0, 1])
+def test_se_spin1_is_expm(imag):
+    inp = case(1, 1, (16,), (0.1,), 1e-3, (0.5,), imag=imag)
+    p, _ = O.derive(inp)
+    o = O.Oracle(p)
+    rng = np.random.default_rng(5)
+    phi = 0.1 * (rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape))
+    o.set_phi(phi)
+    o.call("or_se")
+    got = o.get_phi()
+    cdt = -1j * p.dt if imag else p.dt
+    t1 = p.tau[1]
+    for q in range(16):
+        f = phi[:, 0, 0, q]
+        a1 = t1 * f[1] * np.conj(f[2])
+        b1 = t1 * f[1] * np.conj(f[0])
+        h = np.array([[0, a1, 0], [np.conj(a1), 0, np.conj(b1)], [0, b1, 0]])
+        assert np.allclose(got[:, 0, 0, q], sl.expm(-1j * cdt * h) @ f, atol=1e-15, rtol=1e-12)
+    o.close()
+
+
+@pytest.mark.parametrize("cdt", [1e-3, -1e-3j, 0.3, -0.7j])
+def test_spin2_exponential_is_expm(cdt):
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        c = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+        h = np.zeros((5, 5), complex)
+        for (r, s), val in zip(((0, 1), (0, 2), (1, 2), (2, 3), (2, 4), (3, 4)), c):
+            h[r, s] = val
+            h[s, r] = np.conj(val)
+        v = rng.standard_normal(5) + 1j * rng.standard_normal(5)
+        want = sl.expm(-1j * cdt * h) @ v
+        got = O.expm_herm5(h, complex(cdt), v)
+        assert np.allclose(got, want, atol=1e-12 * np.abs(want).max())
+
+
+# --------------------------------------------- interaction generator vs energy
+def _fvec(spin, f):
+    """(F_x, F_y, F_z) for spinors in the columns of f (ms, npts)."""
+    sx, sy, sz = SPIN_MATS[spin]
+    return [np.real(np.sum(np.conj(f) * (s @ f), axis=0)) for s in (sx, sy, sz)]
+
+
+def _interaction_energy_density(spin, tau, f):
+    """tau0 rho^2/2 + tau1 |F|^2/2 (+ tau2 |Theta|^2/2 for spin-2), per column."""
+    f = np.asarray(f)
+    if f.ndim == 1:
+        f = f[:, None]
+    rho = np.sum(np.abs(f) ** 2, axis=0)
+    fx, fy, fz = _fvec(spin, f)
+    e = tau[0] * rho**2 / 2 + tau[1] * (fx * fx + fy * fy + fz * fz) / 2
+    if spin == 2:
+        th = (2 * f[0] * f[4] - 2 * f[1] * f[3] + f[2] ** 2) / np.sqrt(5)
+        e = e + tau[2] * np.abs(th) ** 2 / 2
+    return e
+
+
+def _interaction_gradient(spin, tau, f, h=1e-6):
+    """dE_int/dphi_m^* = (dE/dRe + i dE/dIm)/2 by central differences."""
+    g = np.zeros_like(f)
+    for m in range(f.shape[0]):
+        for unit in (1.0, 1j):
+            fp, fm = f.copy(), f.copy()
+            fp[m] += h * unit
+            fm[m] -= h * unit
+            d = (_interaction_energy_density(spin, tau, fp) - _interaction_energy_density(spin, tau, fm)) / (2 * h)
+            g[m] += 0.5 * (d if unit == 1.0 else 1j * d)
+    return g
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_se_sp_generator_is_energy_gradient(spin, dim):
+    """SE followed by SP for a tiny real-time step is exp(-i dt H(phi)) with
+    H(phi) phi = V phi + dE_int/dphi^*; so phi^H (H phi) - V rho = 2 E_int
+    (quartic terms are homogeneous of degree 4) at every point."""
+    n = {1: (16,), 2: (16, 16), 3: (16, 16, 16)}[dim]
+    inp = case(dim, spin, n, (0.3,) * dim, 1e-6, (0.4,) * dim, imag=0, natoms=20000, nu=(20.0, 30.0, 40.0))
+    res = {}
+    for sgn in (+1, -1):
+        inp.dt = sgn * 1e-6
+        p, _ = O.derive(inp)
+        o = O.Oracle(p)
+        rng = np.random.default_rng(11)
+        phi = 0.2 * (rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape))
+        o.set_phi(phi)
+        o.call("or_se")
+        o.call("or_sp")
+        res[sgn] = o.get_phi()
+        o.close()
+    dt = 1e-6
+    hphi = 1j * (res[1] - res[-1]) / (2 * dt)
+    # V from the oracle's grid: x_i = (-1 + 2 i/N) L/2, V = sum alpha_d^2 x_d^2
+    alpha = [p.alpha[i] for i in range(3)]
+    axes = [(-1 + 2 * np.arange(p.n[d]) / p.n[d]) * p.n[d] * p.d[d] / 2 if d < dim else np.zeros(1)
+            for d in range(3)]
+    V = (alpha[0] ** 2 * axes[0][None, None, :] ** 2 + alpha[1] ** 2 * axes[1][None, :, None] ** 2 +
+         alpha[2] ** 2 * axes[2][:, None, None] ** 2)
+    tau = [p.tau[i] for i in range(3)]
+    lhs = np.real(np.sum(np.conj(phi) * hphi, axis=0)) - V * np.sum(np.abs(phi) ** 2, axis=0)
+    flat = phi.reshape(phi.shape[0], -1)
+    rhs = 2 * _interaction_energy_density(spin, tau, flat)
+    assert np.allclose(lhs.ravel(), rhs, rtol=1e-6, atol=1e-9 * np.abs(rhs).max())
+
+
+def _se_only(dim, cdt_real, phi, tau):
+    inp = case(dim, 2, {1: (16,), 2: (16, 16), 3: (16, 16, 16)}[dim], (0.3,) * dim, cdt_real, (0.4,) * dim,
+               imag=0, natoms=20000)
+    p, _ = O.derive(inp)
+    for i in range(3):
+        p.tau[i] = tau[i]
+    o = O.Oracle(p)
+    o.set_phi(phi)
+    o.call("or_se")
+    out = o.get_phi()
+    o.close()
+    return out
+
+
+@pytest.mark.parametrize("dim,order", [(1, 2), (2, 3), (3, 3)])
+def test_spin2_se_predictor_factor_from_local_error_order(dim, order):
+    """The spin-2 SE step solves i phi' = Hbar(phi) phi.  With the predictor
+    factor 1 (2D/3D, SRC/cgpe3d.f90:1681) the local error is O(dt^3); with 1/2
+    (1D, SRC/cgpe1d.f90:1443) it is O(dt^2).  The exponent is measured against
+    a tight ODE solve at one point."""
+    tau = (0.0, 40.0, -25.0)
+    shape = (5,) + {1: (1, 1, 16), 2: (1, 16, 16), 3: (16, 16, 16)}[dim]
+    rng = np.random.default_rng(2)
+    phi = np.zeros(shape, complex)
+    f0 = 0.5 * (rng.standard_normal(5) + 1j * rng.standard_normal(5))
+    phi[(slice(None),) + (0,) * 3] = f0
+
+    def rhs(t, y):
+        f = y[:5] + 1j * y[5:]
+        sx, sy, sz = SPIN_MATS[2]
+        fx, fy, fz = [float(v[0]) for v in _fvec(2, f[:, None])]
+        fm = fx - 1j * fy
+        c12 = tau[1] * fm - 0.4 * tau[2] * f[3] * np.conj(f[4])
+        c13 = 0.2 * tau[2] * f[2] * np.conj(f[4])
+        c23 = R15 * tau[1] * fm - 0.2 * tau[2] * f[2] * np.conj(f[3])
+        c34 = R15 * tau[1] * fm - 0.2 * tau[2] * f[1] * np.conj(f[2])
+        c35 = 0.2 * tau[2] * f[0] * np.conj(f[2])
+        c45 = tau[1] * fm - 0.4 * tau[2] * f[0] * np.conj(f[1])
+        h = np.zeros((5, 5), complex)
+        for (r, s), v in zip(((0, 1), (0, 2), (1, 2), (2, 3), (2, 4), (3, 4)), (c12, c13, c23, c34, c35, c45)):
+            h[r, s], h[s, r] = v, np.conj(v)
+        d = -1j * (h @ f)
+        return np.concatenate([d.real, d.imag])
+
+    errs = []
+    dts = [4e-3, 2e-3, 1e-3]
+    for dt in dts:
+        sol = solve_ivp(rhs, (0, dt), np.concatenate([f0.real, f0.imag]), rtol=1e-13, atol=1e-15, method="DOP853")
+        exact = sol.y[:5, -1] + 1j * sol.y[5:, -1]
+        got = _se_only(dim, dt, phi, tau)[(slice(None),) + (0,) * 3]
+        errs.append(np.abs(got - exact).max())
+    slope = np.log(errs[0] / errs[2]) / np.log(dts[0] / dts[2])
+    assert abs(slope - order) < 0.3, (errs, slope)
+
+
+# ---------------------------------------------------- spectral energy check
+def _smooth_state(p, dim, spin, seed):
+    rng = np.random.default_rng(seed)
+    ms = 2 * spin + 1
+    axes = [(-1 + 2 * np.arange(p.n[d]) / p.n[d]) * p.n[d] * p.d[d] / 2 if d < dim else np.zeros(1)
+            for d in range(3)]
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    phi = np.zeros((ms,) + X.shape, complex)
+    for m in range(ms):
+        a = rng.uniform(0.6, 1.0, 3)
+        c = rng.uniform(-0.5, 0.5, 3)
+        k = rng.uniform(-1.0, 1.0, 3)
+        r2 = a[0] * (X - c[0]) ** 2 + (a[1] * (Y - c[1]) ** 2 if dim >= 2 else 0) + \
+            (a[2] * (Z - c[2]) ** 2 if dim >= 3 else 0)
+        ph = k[0] * X + (k[1] * Y if dim >= 2 else 0) + (k[2] * Z if dim >= 3 else 0)
+        phi[m] = rng.uniform(0.5, 1.0) * np.exp(-r2 / 2 + 1j * ph)
+    return phi
+
+
+def _spectral_h(p, dim, spin, phi):
+    """H phi for the propagator's Hamiltonian: k^2 + 2 (g.k).S (3D includes
+    the gz kz Sz part of KE) + V + interactions, derivatives spectral."""
+    sx, sy, sz = SPIN_MATS[spin]
+    ms = 2 * spin + 1
+    ks = [2 * np.pi * np.fft.fftfreq(p.n[d], d=p.d[d]) if d < dim else np.zeros(1) for d in range(3)]
+    KZ, KY, KX = np.meshgrid(ks[2], ks[1], ks[0], indexing="ij")
+    axes = list(range(1, 4))[3 - dim:]
+    ph = np.fft.fftn(phi, axes=axes)
+    hk = (KX**2 + KY**2 + KZ**2)[None] * ph
+    g = [p.gamma[i] for i in range(3)]
+    for a in range(ms):
+        for b in range(ms):
+            m = 2 * (g[0] * KX * sx[a, b] + g[1] * KY * sy[a, b] + g[2] * KZ * sz[a, b])
+            hk[a] += m * ph[b]
+    return np.fft.ifftn(hk, axes=axes)
+
+
+def _spectral_energy_mu(p, dim, spin, phi):
+    hphi = _spectral_h(p, dim, spin, phi)
+    dv = np.prod([p.d[i] for i in range(dim)])
+    rho = np.sum(np.abs(phi) ** 2, axis=0)
+    flat = phi.reshape(phi.shape[0], -1)
+    tau = [p.tau[i] for i in range(3)]
+    eint = _interaction_energy_density(spin, tau, flat).reshape(rho.shape)
+    axes = [(-1 + 2 * np.arange(p.n[k]) / p.n[k]) * p.n[k] * p.d[k] / 2 if k < dim else np.zeros(1)
+            for k in range(3)]
+    V = (p.alpha[0] ** 2 * axes[0][None, None, :] ** 2 + p.alpha[1] ** 2 * axes[1][None, :, None] ** 2 +
+         p.alpha[2] ** 2 * axes[2][:, None, None] ** 2)
+    e_lin = np.real(np.sum(np.conj(phi) * hphi, axis=0))
+    en = (np.sum(e_lin + V * rho + eint) * dv) / (np.sum(rho) * dv)
+    # mu_m N_m = int Re(phi_m^* [(H_lin phi)_m + V phi_m + dE_int/dphi_m^*])
+    grad = _interaction_gradient(spin, tau, flat).reshape(phi.shape)
+    num = np.real(np.conj(phi) * (hphi + V[None] * phi + grad))
+    nm = np.sum(np.abs(flat) ** 2, axis=1) * dv
+    mu = np.sum(num.reshape(phi.shape[0], -1), axis=1) * dv / nm
+    szd = np.real(np.diag(SPIN_MATS[spin][2]))
+    mz = np.sum(szd[:, None] * np.abs(flat) ** 2) * dv
+    return en, mu, mz, nm
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_energy_matches_spectral_expectation(spin, dim):
+    """ENERGY's EN, MU, MZ, NORM against <phi|H|phi> with the propagator's
+    Hamiltonian H = k^2 + 2(g.k).S + V + interactions (spectral derivatives).
+    The stencil/Simpson discretisation error (~1e-6 here) is common to the SO
+    coupling on and off, so the SO-coupling contribution E(g) - E(0) is
+    compared to a much tighter bar."""
+    n = {2: (64, 64), 3: (48, 48, 48)}[dim]
+    d = {2: (0.25, 0.25), 3: (0.3, 0.3, 0.3)}[dim]
+    gam = {2: (0.7, 1.1), 3: (0.5, 0.8, 0.6)}[dim]
+    res = {}
+    for g in (gam, (0.0,) * dim):
+        inp = case(dim, spin, n, d, 1e-3, g, imag=1, natoms=2000, nu=(20.0, 23.0, 27.0))
+        p, _ = O.derive(inp)
+        o = O.Oracle(p)
+        phi = _smooth_state(p, dim, spin, seed=dim * 10 + spin)
+        o.set_phi(phi)
+        en, mu, mz = o.energy()
+        norm = o.normc()
+        o.close()
+        res[g] = (en, mu, mz, norm) + _spectral_energy_mu(p, dim, spin, phi)
+    en, mu, mz, norm, e_s, mu_s, mz_s, n_s = res[gam]
+    assert abs(en - e_s) < 1e-5 * abs(e_s)
+    assert np.allclose(mu, mu_s, rtol=1e-5)
+    assert abs(mz - mz_s) < 1e-6 * max(1.0, abs(mz_s))
+    assert np.allclose(norm, n_s, rtol=1e-6)
+    e0, mu0 = res[(0.0,) * dim][0], res[(0.0,) * dim][1]
+    e0s, mu0s = res[(0.0,) * dim][4], res[(0.0,) * dim][5]
+    d_or, d_sp = en - e0, e_s - e0s
+    assert abs(d_sp) > 1e-3 * abs(e_s)  # the SO coupling matters in this state
+    assert abs(d_or - d_sp) < 1e-6 * abs(e_s)
+    assert np.allclose(mu - mu0, mu_s - mu0s, atol=1e-6 * np.abs(mu_s).max())
