@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Throughput of the split-step propagator on one MI355X.
+
+Workload (one "step" = one full time step of the reference loop,
+SRC/cgpe3d.f90:452-495): 3D spin-2 real-time dynamics on a 256^3 grid,
+DX=DY=DZ=0.1, DT=0.0005, 87Rb spin-2 (A0/A2/A4 = 87.93/91.28/99.18 a_B),
+NATOMS=1e5, trap 20/20/20 Hz, GAMMAX=GAMMAY=GAMMAZ=0.5, FP32 propagation with
+FP64 norm/energy reductions, observables (ENERGY, RAD, NORMC) every STP=100
+steps inside the timed region.  Initial state: an imaginary-time relaxation
+of the reference's Gaussian guess (FP32, DT=0.005, --prep-steps steps) plus
+seeded 1% complex Gaussian noise phi += 0.01 max|phi| (xi_r + i xi_i)/sqrt(2).
+Synthetic parameters; no data is read.
+
+Prints one JSON line.  value = steps/s of the whole job (one GPU).  The
+roofline object is the dominant kernel's algorithmic HBM bytes per launch
+(2*S, S = N*MS*8 B: every pass reads and writes the whole spinor once) over
+its average launch time measured with HIP events on the library's stream.
+cpu_baseline: the FP64 CPU restatement (oracle/, OpenMP) timed on a bounded
+sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3D spin-2 256^3 steps/s (%HBM roofline); time-to-TOL, surrogate vs plain guess"
+PEAK_GBS = 8000.0
+
+
+def workload(n, imag, dt):
+    from fortress_amd.abi import make_input
+    return make_input(3, 2, (n, n, n), (0.1, 0.1, 0.1), dt, 87.0, (87.93, 91.28, 99.18), 100000,
+                      (20.0, 20.0, 20.0), (0.5, 0.5, 0.5), switch_im=imag, stp=100)
+
+
+def noisy(phi, seed, amp=0.01):
+    rng = np.random.default_rng(seed)
+    scale = amp * np.abs(phi).max() / np.sqrt(2.0)
+    out = np.empty_like(phi)
+    for m in range(phi.shape[0]):  # per component to bound host memory
+        out[m] = phi[m] + scale * (rng.standard_normal(phi.shape[1:]) + 1j * rng.standard_normal(phi.shape[1:]))
+    return out
+
+
+def cpu_baseline(phi0, n, steps):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as O
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    O.lib().or_set_threads(threads)
+    p, _ = O.derive(workload(n, 0, 5e-4))
+    o = O.Oracle(p)
+    o.set_phi(phi0)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "%d FP64 steps of the same %d^3 spin-2 real-time workload and initial state "
+                      "(oracle/fgpe_oracle.c, OpenMP, %d threads), %.1f s" % (steps, n, threads, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--prep-steps", type=int, default=1500)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+                 "(the slab decomposition over RCCL is design-only); use --gpus 1")
+
+    from fortress_amd import fgpe
+    from fortress_amd.abi import FGPE_FP32
+
+    stp = 100
+    # --- initial state: imaginary-time relaxation of the Gaussian guess + 1% noise
+    p_im, _ = fgpe.derive(workload(args.n, 1, 0.005), FGPE_FP32)
+    with fgpe.Solver(p_im) as s:
+        s.initial_guess()
+        s.normalize()
+        s.run(args.prep_steps)
+        prep_en = s.observables()[0]
+        phi_g = s.download()
+    phi0 = noisy(phi_g, seed=1234)
+    del phi_g
+
+    p, _ = fgpe.derive(workload(args.n, 0, 5e-4), FGPE_FP32)
+    s = fgpe.Solver(p)
+    s.upload(phi0)
+    s.observables()
+
+    def advance(k):
+        done = 0
+        while done < k:
+            c = min(stp, k - done)
+            s.run(c)
+            done += c
+            if done % stp == 0:
+                s.observables()
+
+    advance(args.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    advance(args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    steps_per_s = args.steps / elapsed
+    en, mu, mz, rms = s.observables()
+
+    # --- per-kernel times (HIP events on the library stream)
+    passes = s.pass_times(50)
+    s.close()
+    S = args.n ** 3 * 5 * 8
+    b_pass = 2 * S  # algorithmic bytes per launch of every line pass
+    dom = max((k for k in passes if k in ("x", "y", "k")), key=lambda k: passes[k][0])
+    dom_ms = passes[dom][0]
+    achieved = b_pass / (dom_ms * 1e-3) / 1e9
+    kernel_names = {"x": "k_xpass (inverse-x + SE + SP + NORMC + forward-x)", "y": "k_ypass (y FFT, one component)",
+                    "k": "k_kpass (forward-z + KE + SOC + inverse-z)"}
+    per_pass = {k: {"ms": round(v[0], 4), "launches": v[1],
+                    "GBps": round(b_pass / (v[0] * 1e-3) / 1e9, 1) if k in ("x", "y", "k") else None}
+                for k, v in passes.items()}
+    b_alg = 2 * 4 * S  # four read+write passes per step
+    line = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / steps_per_s, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 propagation, fp64 reductions",
+        "data": "synthetic: imaginary-time relaxed Gaussian guess + seeded 1% complex noise",
+        "config": {"workload": "3D spin-2 real-time split-step, %d^3, DX=0.1, DT=0.0005, 87Rb spin-2, N=1e5, "
+                               "trap 20/20/20 Hz, gamma=(0.5,0.5,0.5), STP=100 observables in the timed region"
+                               % args.n,
+                   "grid": [args.n] * 3, "components": 5, "stp": stp, "prep": {"imag_steps": args.prep_steps,
+                                                                              "dt": 0.005, "energy": prep_en}},
+        "roofline": {"bound": "hbm", "kernel": kernel_names[dom], "achieved": round(achieved, 1),
+                     "peak": PEAK_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": b_pass},
+        "step_roofline": {"bytes_per_step": b_alg, "achieved_GBps": round(b_alg * steps_per_s / 1e9, 1),
+                          "frac": round(b_alg * steps_per_s / 1e9 / PEAK_GBS, 4)},
+        "passes": per_pass,
+        "final_energy": en,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(phi0, args.n, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -108,13 +108,18 @@ template <typename T> C<T> cphase(double a, double dt, int imag) {
 
 template <typename T> int build_tables(fgpe_handle* h) {
   const fgpe_params& p = h->p;
-  const int Q = Cfg<T>::Q;
+  // each axis' twiddle tables follow the points-per-thread of the pass that
+  // transforms it: x by pass A (or the q1D kernel), y by B/D in 3D and by the
+  // k-space pass in 2D, z by the k-space pass
+  const int qx = h->dim == 1 ? Cfg<T>::Q1 : Cfg<T>::QX;
+  const int qy = h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK;
+  const int qz = Cfg<T>::QK;
   auto kx = kgrid(h->nx, p.d[0]), ky = kgrid(h->ny, p.d[1]), kz = kgrid(h->nz, p.d[2]);
   auto xx = xgrid(h->nx, p.d[0]), yy = xgrid(h->ny, p.d[1]), zz = xgrid(h->nz, p.d[2]);
   int rc;
-  if ((rc = upload_vec(h, &h->twx, stage_twiddles<T>(h->nx, Q)))) return rc;
-  if ((rc = upload_vec(h, &h->twy, stage_twiddles<T>(h->ny > 1 ? h->ny : Q, Q)))) return rc;
-  if ((rc = upload_vec(h, &h->twz, stage_twiddles<T>(h->nz > 1 ? h->nz : Q, Q)))) return rc;
+  if ((rc = upload_vec(h, &h->twx, stage_twiddles<T>(h->nx, qx)))) return rc;
+  if ((rc = upload_vec(h, &h->twy, stage_twiddles<T>(h->ny > 1 ? h->ny : qy, qy)))) return rc;
+  if ((rc = upload_vec(h, &h->twz, stage_twiddles<T>(h->nz > 1 ? h->nz : qz, qz)))) return rc;
   const double dt = p.dt;
   const int imag = p.imag;
   // KE factors: exp(-i CDT kx^2) * exp(-i CDT ky^2) * line factor
@@ -216,6 +221,28 @@ int ensure_partials(fgpe_handle* h, size_t n) {
 }
 
 
+int prof_begin(fgpe_handle* h) {
+  if (!h->profiling) return 0;
+  if (h->ev_used + 2 > h->ev_pool.size()) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      h->profiling = false;
+      return fail(FGPE_EDEVICE, "hipEventCreate failed while profiling");
+    }
+    h->ev_pool.push_back(e0);
+    h->ev_pool.push_back(e1);
+  }
+  (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
+  return 0;
+}
+
+void prof_end(fgpe_handle* h, int kind) {
+  if (!h->profiling) return;
+  (void)hipEventRecord(h->ev_pool[h->ev_used + 1], h->stream);
+  h->ev_used += 2;
+  h->ev_kind.push_back(kind);
+}
+
 int grid_for(long n, int threads) {
   long b = (n + threads - 1) / threads;
   return (int)std::min<long>(std::max<long>(b, 1), 8192);
@@ -301,7 +328,6 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
     return fail(FGPE_EINVAL, "dim must be 1..3 and spin 1 or 2");
   if (p->precision != FGPE_FP32 && p->precision != FGPE_FP64) return fail(FGPE_EINVAL, "bad precision");
   const bool fp32 = p->precision == FGPE_FP32;
-  const int Q = fp32 ? Cfg<float>::Q : Cfg<double>::Q;
   const int TX = fp32 ? Cfg<float>::TX : Cfg<double>::TX;
   const int nx = p->n[0], ny = p->dim >= 2 ? p->n[1] : 1, nz = p->dim >= 3 ? p->n[2] : 1;
   if (p->dim == 1) {
@@ -311,7 +337,6 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
       return fail(FGPE_EINVAL, "grid %dx%dx%d not supported (each axis 32..512, power of 2)", nx, ny, nz);
     if (nx % TX) return fail(FGPE_EINVAL, "NX must be a multiple of %d", TX);
   }
-  (void)Q;
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev < 1) return fail(FGPE_EDEVICE, "no HIP device available (%s)", hipGetErrorString(e));
@@ -356,6 +381,8 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
   if ((rc = dalloc(h, (void**)&h->dnorm, 8 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->dsig, 2 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->derr, sizeof(int)))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->ticket, sizeof(unsigned int)))) return bail(rc);
+  if (hipMemset(h->ticket, 0, sizeof(unsigned int)) != hipSuccess) return bail(fail(FGPE_EDEVICE, "hipMemset failed"));
   const double sig0[2] = {20.0, 20.0};  // SRC/cgpe3d.f90:443-444
   if (hipMemcpy(h->dsig, sig0, sizeof sig0, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(FGPE_EDEVICE, "hipMemcpy failed"));
@@ -371,6 +398,7 @@ void fgpe_destroy(fgpe_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (void* p : h->allocs) (void)hipFree(p);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -493,6 +521,29 @@ int fgpe_center_abs(fgpe_handle* h, double* absphi) {
 int fgpe_sync(fgpe_handle* h) {
   if (!h) return fail(FGPE_EINVAL, "null handle");
   HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fgpe_pass_times(fgpe_handle* h, int64_t nsteps, double* ms_out, int32_t* count_out) {
+  if (!h || !ms_out || !count_out) return fail(FGPE_EINVAL, "null argument");
+  h->profiling = true;
+  h->ev_used = 0;
+  h->ev_kind.clear();
+  int rc = fgpe_run(h, nsteps);
+  h->profiling = false;
+  if (rc) return rc;
+  double sum[FGPE_NPASS] = {0};
+  int cnt[FGPE_NPASS] = {0};
+  for (size_t i = 0; i < h->ev_kind.size(); ++i) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, h->ev_pool[2 * i], h->ev_pool[2 * i + 1]));
+    sum[h->ev_kind[i]] += t;
+    cnt[h->ev_kind[i]] += 1;
+  }
+  for (int k = 0; k < FGPE_NPASS; ++k) {
+    ms_out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    count_out[k] = cnt[k];
+  }
   return 0;
 }
 

@@ -12,7 +12,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace fgpe {
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).  Used
+// wherever a register array is indexed, so the index is a constant and the
+// array stays in VGPRs (a rolled loop would demote it to scratch memory).
+template <int B, int E> struct SFor {
+  template <typename F> __host__ __device__ __forceinline__ static void run(F&& f) {
+    if constexpr (B < E) {
+      f(std::integral_constant<int, B>{});
+      SFor<B + 1, E>::run(f);
+    }
+  }
+};
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<float> { using type = float2; };
@@ -183,6 +197,37 @@ template <int DIR, typename C> struct Dft<8, DIR, C> : DftAB<2, 4, DIR, C> {};
 template <int DIR, typename C> struct Dft<12, DIR, C> : DftAB<4, 3, DIR, C> {};
 template <int DIR, typename C> struct Dft<16, DIR, C> : DftAB<4, 4, DIR, C> {};
 
+// ------------------------------------------------------- buffer accesses
+// One component of the state as a raw buffer resource: 4 SGPRs of descriptor,
+// a 32-bit per-lane byte offset (VGPR) and a uniform byte offset (SGPR or
+// immediate).  Strided line loads then cost one VGPR for the address instead
+// of a 64-bit pointer per element, and accesses past the component's size
+// read zero / are dropped by the hardware.
+template <typename C> struct CompBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ CompBuf(const C* base, long n) {
+    unsigned long bytes = (unsigned long)n * sizeof(C);
+    unsigned int nrec = bytes > 0xffffffffUL ? 0xffffffffU : (unsigned int)bytes;
+    r = __builtin_amdgcn_make_buffer_rsrc(const_cast<C*>(base), (short)0, (int)nrec, 0x00020000);
+  }
+  __device__ __forceinline__ C ld(unsigned int voff, unsigned int soff) const {
+    if constexpr (sizeof(C) == 8) {
+      return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    } else {
+      return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    }
+  }
+  __device__ __forceinline__ void st(C v, unsigned int voff, unsigned int soff) const {
+    if constexpr (sizeof(C) == 8) {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, voff, soff, 0);
+    } else {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, voff, soff, 0);
+    }
+  }
+};
+
 // ------------------------------------------------------------ line FFT plan
 // radix of the stage that leaves `rem` points to combine, given Q points per
 // thread: the largest of 16,12,8,6,4,3,2 dividing both.
@@ -222,41 +267,51 @@ template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
   static_assert(R > 0, "unsupported FFT size / points-per-thread");
   __device__ __forceinline__ static void run(C (&v)[Q], C* lds, const C* tw, int j) {
     C x[Q];
-#pragma unroll
-    for (int u = 0; u < Q / R; ++u) {
+    SFor<0, Q / R>::run([&](auto ui) {
+      constexpr int u = decltype(ui)::value;
       const int b = j + u * P;
       const int k = b % NS;
       C y[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) y[r] = v[u + r * (Q / R)];
+      SFor<0, R>::run([&](auto ri) {
+        constexpr int r = decltype(ri)::value;
+        y[r] = v[u + r * (Q / R)];
+      });
       if constexpr (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
+        SFor<1, R>::run([&](auto ri) {
+          constexpr int r = decltype(ri)::value;
           C w = tw[r * NS + k];
           if constexpr (DIR > 0) w.y = -w.y;
           y[r] = cmul(y[r], w);
-        }
+        });
       }
       Dft<R, DIR, C>::run(y);
-#pragma unroll
-      for (int r = 0; r < R; ++r) x[u + r * (Q / R)] = y[r];
-    }
+      SFor<0, R>::run([&](auto ri) {
+        constexpr int r = decltype(ri)::value;
+        x[u + r * (Q / R)] = y[r];
+      });
+    });
     if constexpr (LAST) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) v[q] = x[q];
+      SFor<0, Q>::run([&](auto qi) {
+        constexpr int q = decltype(qi)::value;
+        v[q] = x[q];
+      });
     } else {
       __syncthreads();  // previous readers of this LDS line are done
-#pragma unroll
-      for (int u = 0; u < Q / R; ++u) {
+      SFor<0, Q / R>::run([&](auto ui) {
+        constexpr int u = decltype(ui)::value;
         const int b = j + u * P;
         const int k = b % NS;
         const int idxd = (b / NS) * NS * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[lpad(idxd + r * NS)] = x[u + r * (Q / R)];
-      }
+        SFor<0, R>::run([&](auto ri) {
+          constexpr int r = decltype(ri)::value;
+          lds[lpad(idxd + r * NS)] = x[u + r * (Q / R)];
+        });
+      });
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < Q; ++q) v[q] = lds[lpad(j + P * q)];
+      SFor<0, Q>::run([&](auto qi) {
+        constexpr int q = decltype(qi)::value;
+        v[q] = lds[lpad(j + P * q)];
+      });
       FftStage<N, Q, NS * R, DIR, C>::run(v, lds, tw + (NS > 1 ? NS * R : 0), j);
     }
   }
@@ -283,6 +338,8 @@ struct Phys {
 
 __host__ __device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
 __host__ __device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
+__host__ __device__ __forceinline__ float exp_t(float a) { return expf(a); }
+__host__ __device__ __forceinline__ double exp_t(double a) { return exp(a); }
 
 // exp(-i CDT e) for real e: real time exp(-i DT e), imaginary time exp(-DT e)
 template <typename C, typename T> __host__ __device__ __forceinline__ C phase_of(T e, T dt, int imag) {
@@ -297,49 +354,64 @@ template <typename C, typename T> __host__ __device__ __forceinline__ C mul_micd
   return imag ? mk<C>(-dt * z.x, -dt * z.y) : mk<C>(dt * z.y, -dt * z.x);
 }
 
-// SP: diagonal trap + density phases (SRC/cgpe3d.f90:1023-1063)
+// exp(-i CDT theta/DT) for a phase theta = DT*E formed in FP64: the argument
+// is reduced exactly in FP64 (mod 2 pi, or by multiples of ln 2 for the
+// imaginary-time exponential) and only the small remainder is evaluated in
+// the state's precision, so a large DT*E (stiff, dense states) costs no more
+// than one rounding of the result.
+template <typename C> __host__ __device__ __forceinline__ C phase_from64(double theta, int imag) {
+  using T = decltype(C::x);
+  if (imag) {
+    const double x = -theta;
+    const double n = rint(x * 1.44269504088896340736);  // 1/ln2
+    const double r = fma(-n, 0.69314718055994530942, x);
+    return mk<C>((T)ldexp((double)exp_t((T)r), (int)n), (T)0);
+  }
+  const double k = rint(theta * 0.15915494309189533577);  // 1/(2 pi)
+  const double r = fma(-k, 6.28318530717958623200, theta) - k * 2.44929359829470635445e-16;
+  T s, c;
+  sincos_t((T)r, &s, &c);
+  return mk<C>(c, -s);
+}
+
+// SP: diagonal trap + density phases (SRC/cgpe3d.f90:1023-1063).  The
+// densities, E_m and the phase DT*E_m are formed in FP64 from the stored
+// amplitudes; the multiply stays in the state's precision.
 template <int SPIN, typename C, typename T>
 __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T dt) {
-  const T t0 = (T)p.tau0, t1 = (T)p.tau1, t2 = (T)p.tau2;
+  const double t0 = p.tau0, t1 = p.tau1, t2 = p.tau2, dtd = p.dt;
+  (void)dt;
   if constexpr (SPIN == 1) {
-    T a0 = cabs2(ph[0]), a1 = cabs2(ph[1]), a2 = cabs2(ph[2]);
-    T rho = a0 + a1 + a2;
-    T base = v + t0 * rho;
-    T e[3] = {base + t1 * (a0 + a1 - a2), base + t1 * (a0 + a2), base + t1 * (a2 + a1 - a0)};
-#pragma unroll
-    for (int m = 0; m < 3; ++m) ph[m] = cmul(ph[m], phase_of<C, T>(e[m], dt, p.imag));
+    double a[3];
+    for (int m = 0; m < 3; ++m) a[m] = (double)ph[m].x * (double)ph[m].x + (double)ph[m].y * (double)ph[m].y;
+    const double rho = a[0] + a[1] + a[2];
+    const double base = (double)v + t0 * rho;
+    const double e[3] = {base + t1 * (a[0] + a[1] - a[2]), base + t1 * (a[0] + a[2]), base + t1 * (a[2] + a[1] - a[0])};
+    for (int m = 0; m < 3; ++m) ph[m] = cmul(ph[m], phase_from64<C>(dtd * e[m], p.imag));
   } else {
-    T a[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) a[m] = cabs2(ph[m]);
-    T rho = a[0] + a[1] + a[2] + a[3] + a[4];
-    T fz = (T)2 * a[0] + a[1] - a[3] - (T)2 * a[4];
-    T base = v + t0 * rho;
-    T e[5] = {base + (T)2 * t1 * fz + (T)0.4 * t2 * a[4], base + t1 * fz + (T)0.4 * t2 * a[3],
-              base + (T)0.2 * t2 * a[2], base - t1 * fz + (T)0.4 * t2 * a[1],
-              base - (T)2 * t1 * fz + (T)0.4 * t2 * a[0]};
-#pragma unroll
-    for (int m = 0; m < 5; ++m) ph[m] = cmul(ph[m], phase_of<C, T>(e[m], dt, p.imag));
+    double a[5];
+    for (int m = 0; m < 5; ++m) a[m] = (double)ph[m].x * (double)ph[m].x + (double)ph[m].y * (double)ph[m].y;
+    const double rho = a[0] + a[1] + a[2] + a[3] + a[4];
+    const double fz = 2.0 * a[0] + a[1] - a[3] - 2.0 * a[4];
+    const double base = (double)v + t0 * rho;
+    const double e[5] = {base + 2.0 * t1 * fz + 0.4 * t2 * a[4], base + t1 * fz + 0.4 * t2 * a[3],
+                         base + 0.2 * t2 * a[2], base - t1 * fz + 0.4 * t2 * a[1],
+                         base - 2.0 * t1 * fz + 0.4 * t2 * a[0]};
+    for (int m = 0; m < 5; ++m) ph[m] = cmul(ph[m], phase_from64<C>(dtd * e[m], p.imag));
   }
 }
 
-// F_x - i F_y and F_z of a spin-2 spinor with the reference's SX, SY, SZ
-// (SRC/cgpe3d.f90:686-696,1086-1088), then C12..C45 (SRC/cgpe3d.f90:1112-1122)
+// C12..C45 of SRC/cgpe3d.f90:1112-1122 at one point.  F_- = F_x - i F_y with
+// the reference's SX, SY (SRC/cgpe3d.f90:686-696, standard spin-2 matrices)
+// equals phi^H S_- phi, S_- = SX - i SY the lowering operator, i.e.
+// 2 phi1 conj(phi2) + sqrt6 phi2 conj(phi3) + sqrt6 phi3 conj(phi4) + 2 phi4 conj(phi5)
+// (1-based components; FXYZ/MAT_C at SRC/cgpe3d.f90:1086-1087,1676).
 template <typename C, typename T> __host__ __device__ __forceinline__ void mat_c2(const C* ph, const Phys& p, C* c) {
   const T r = (T)1.22474487139158904910;  // sqrt(1.5)
-  // F_- = sum conj(phi) (SX - i SY) phi  restricted to its real parts:
-  // FX = Re(phi^H SX phi), FY = Re(phi^H SY phi)
-  C sx[5] = {ph[1], cadd(ph[0], cscale(ph[2], r)), cscale(cadd(ph[1], ph[3]), r), cadd(cscale(ph[2], r), ph[4]), ph[3]};
-  T fx = 0, fy = 0;
-#pragma unroll
-  for (int m = 0; m < 5; ++m) fx += ph[m].x * sx[m].x + ph[m].y * sx[m].y;
-  // SY phi = i * {-p1, p0 - r p2, r p1 - r p3, r p2 - p4, p3}
-  C sy[5] = {cmuli(mk<C>(-ph[1].x, -ph[1].y)), cmuli(csub(ph[0], cscale(ph[2], r))),
-             cmuli(cscale(csub(ph[1], ph[3]), r)), cmuli(csub(cscale(ph[2], r), ph[4])), cmuli(ph[3])};
-#pragma unroll
-  for (int m = 0; m < 5; ++m) fy += ph[m].x * sy[m].x + ph[m].y * sy[m].y;
+  const C fm0 = cadd(cadd(cmulc(ph[0], ph[1]), cmulc(ph[3], ph[4])),
+                     cscale(cadd(cmulc(ph[1], ph[2]), cmulc(ph[2], ph[3])), r));
   const T t1 = (T)p.tau1, t2 = (T)p.tau2;
-  C fm = mk<C>(t1 * fx, -t1 * fy);  // tau1 * (FX - i FY)
+  const C fm = cscale(fm0, (T)2 * t1);  // tau1 * F_-
   c[0] = csub(fm, cscale(cmulc(ph[3], ph[4]), (T)0.4 * t2));
   c[1] = cscale(cmulc(ph[2], ph[4]), (T)0.2 * t2);
   c[2] = csub(cscale(fm, r), cscale(cmulc(ph[2], ph[3]), (T)0.2 * t2));

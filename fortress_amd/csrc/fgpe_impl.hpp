@@ -30,13 +30,22 @@ int fail(int code, const char* fmt, ...);
 constexpr double PI = 3.141592653589793238462643383279502884;
 constexpr double EPS = 1.0e-40;
 
+// Points per thread (Q) of each line pass and the x-width (TX) of the
+// strided-line tiles.  Q sets the register footprint: the all-component
+// passes hold 2*SPIN+1 complex values per point in registers.
 template <typename T> struct Cfg;
 template <> struct Cfg<float> {
-  static constexpr int Q = 16;   // points per thread in a line
+  static constexpr int QX = 8;  // pass A (x lines, all components)
+  static constexpr int QY = 16;  // passes B/D (y lines, one component)
+  static constexpr int QK = 8;   // pass C (k-space lines, all components)
+  static constexpr int Q1 = 16;  // q1D whole-step kernel
   static constexpr int TX = 16;  // x columns per strided-line block (128 B rows)
 };
 template <> struct Cfg<double> {
-  static constexpr int Q = 8;
+  static constexpr int QX = 8;
+  static constexpr int QY = 8;
+  static constexpr int QK = 4;
+  static constexpr int Q1 = 8;
   static constexpr int TX = 8;
 };
 
@@ -68,10 +77,16 @@ struct fgpe_handle {
   double* dnorm = nullptr;       // [8]: norms, TNORM
   double* dsig = nullptr;        // Newton state
   int* derr = nullptr;
+  unsigned int* ticket = nullptr;  // last-block counter of pass A
   double* partials = nullptr;    // reduction scratch
   size_t partials_n = 0;
   double last_ms = 0;
   bool timed = false;
+  // per-kernel timing (fgpe_pass_times): events recorded around launches
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kind;
   std::vector<void*> allocs;
 };
 
@@ -106,6 +121,10 @@ template <typename T> inline fgpe::Tables<T> tables(fgpe_handle* h) {
 }
 
 
+
+// begin/end markers of one timed kernel launch (no-ops unless profiling)
+int prof_begin(fgpe_handle* h);
+void prof_end(fgpe_handle* h, int kind);
 
 // implemented per (T, SPIN) in fgpe_ops_*.hip
 template <typename T, int SPIN> int run_steps(fgpe_handle* h, long nsteps);

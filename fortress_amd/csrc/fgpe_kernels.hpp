@@ -42,212 +42,6 @@ struct Geom {
   long npts;
 };
 
-// ----------------------------------------------------------- reductions
-// butterfly sum over the active lanes of a wave; blocks narrower than a wave
-// (small 1D lines) only exchange among their own lanes
-template <int M> __device__ __forceinline__ void wave_sum(double (&a)[M]) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
-    if (off < (int)blockDim.x)
-#pragma unroll
-      for (int m = 0; m < M; ++m) a[m] += __shfl_xor(a[m], off, 64);
-}
-
-// sum a[M] over the block and write out[blockIdx.x*M + m]; red: >= 16*M doubles
-template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M], double* red, double* out) {
-  wave_sum<M>(a);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int m = 0; m < M; ++m) red[wid * M + m] = a[m];
-  __syncthreads();
-  if (threadIdx.x < M) {
-    double s = 0;
-    for (int w = 0; w < nw; ++w) s += red[w * M + threadIdx.x];
-    out[(long)blockIdx.x * M + threadIdx.x] = s;
-  }
-}
-
-// ------------------------------------------------------------ pass A (x)
-// Lines along x, all components.  flags: XF_INV inverse FFT first, XF_PHYS
-// real-space physics (1/N, SE, SP, NORMC partials), XF_FWD forward FFT last,
-// XF_SCALE multiply by the pending scale on load.
-template <typename T, int SPIN, int N, int Q>
-__global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
-                                               double* __restrict__ partials) {
-  using C = C_t<T>;
-  constexpr int MS = 2 * SPIN + 1;
-  constexpr int P = N / Q;
-  constexpr int LS = line_stride(N);
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
-  double* red = reinterpret_cast<double*>(lds_tw + TWS);
-  C* lds = reinterpret_cast<C*>(red + 16 * MS);
-
-  const int lpw = blockDim.x / P;
-  const int tl = threadIdx.x / P, j = threadIdx.x % P;
-  const long line = (long)blockIdx.x * lpw + tl;
-  const long nlines = (long)g.ny * g.nz;
-  const bool valid = line < nlines;
-  const int y = valid ? (int)(line % g.ny) : 0, z = valid ? (int)(line / g.ny) : 0;
-  C* my = lds + tl * LS;
-
-  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
-
-  C v[MS][Q];
-  const long base = line * N;
-#pragma unroll
-  for (int m = 0; m < MS; ++m) {
-    C sc = mk<C>((T)1, (T)0);
-    if (flags & XF_SCALE) sc = tb.cscale[m];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      C a = valid ? phi[m * g.npts + base + j + P * q] : mk<C>((T)0, (T)0);
-      v[m][q] = (flags & XF_SCALE) ? cmul(a, sc) : a;
-    }
-  }
-  __syncthreads();
-  if (flags & XF_INV) {
-#pragma unroll
-    for (int m = 0; m < MS; ++m) line_fft<N, Q, +1>(v[m], my, lds_tw, j);
-  }
-  if (flags & XF_PHYS) {
-    const T dt = (T)ph.dt;
-    const T inv = (T)ph.inv_npts;
-    const T vyz = tb.vy[y] + tb.vz[z];
-    const double wyz = tb.wy[y] * tb.wz[z];
-    double acc[MS];
-#pragma unroll
-    for (int m = 0; m < MS; ++m) acc[m] = 0.0;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int x = j + P * q;
-      C pt[MS];
-#pragma unroll
-      for (int m = 0; m < MS; ++m) pt[m] = cscale(v[m][q], inv);
-      if constexpr (SPIN == 1)
-        se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
-      else
-        se2_point<C, T>(pt, ph, dt, ph.dim == 1 ? (T)0.5 : (T)1, sizeof(T) == 4 ? (T)1e-8 : (T)1e-17);
-      sp_point<SPIN, C, T>(pt, tb.vx[x] + vyz, ph, dt);
-      const double wx = tb.wx[x];
-#pragma unroll
-      for (int m = 0; m < MS; ++m) {
-        v[m][q] = pt[m];
-        acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < MS; ++m) acc[m] = valid ? acc[m] * wyz : 0.0;
-    block_sum_store<MS>(acc, red, partials);
-  }
-  if (flags & XF_FWD) {
-#pragma unroll
-    for (int m = 0; m < MS; ++m) line_fft<N, Q, -1>(v[m], my, lds_tw, j);
-  }
-  if (valid) {
-#pragma unroll
-    for (int m = 0; m < MS; ++m)
-#pragma unroll
-      for (int q = 0; q < Q; ++q) phi[m * g.npts + base + j + P * q] = v[m][q];
-  }
-}
-
-// ------------------------------------------------------- pass B/D (y)
-// Lines along y for one component; a block owns TX consecutive x at one z.
-// dir -1 forward, +1 inverse; scale applies the pending renormalisation.
-template <typename T, int N, int Q, int TX>
-__global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale) {
-  using C = C_t<T>;
-  constexpr int P = N / Q;
-  constexpr int LS = line_stride(N);
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
-  C* lds = lds_tw + TWS;
-  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
-  const int x = blockIdx.x * TX + l;
-  const int z = blockIdx.y, m = blockIdx.z;
-  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twy[i];
-  C* base = phi + m * g.npts + (long)z * g.nx * g.ny + x;
-  C v[Q];
-  C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    C a = base[(long)(j + P * q) * g.nx];
-    v[q] = scale ? cmul(a, sc) : a;
-  }
-  __syncthreads();
-  if (dir < 0)
-    line_fft<N, Q, -1>(v, lds + l * LS, lds_tw, j);
-  else
-    line_fft<N, Q, +1>(v, lds + l * LS, lds_tw, j);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) base[(long)(j + P * q) * g.nx] = v[q];
-}
-
-// ---------------------------------------------- pass C (k-space lines)
-// Lines along the last axis (z in 3D, y in 2D), all components: forward FFT,
-// KE (SRC/cgpe3d.f90:875-899), SOC (904-1004), inverse FFT.  A block owns TX
-// consecutive x (and one y in 3D).
-template <typename T, int SPIN, int DIM, int N, int Q, int TX>
-__global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
-  using C = C_t<T>;
-  constexpr int MS = 2 * SPIN + 1;
-  constexpr int P = N / Q;
-  constexpr int LS = line_stride(N);
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
-  C* lds_ke = lds_tw + TWS;
-  C* lds = lds_ke + MS * N;
-  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
-  const int x = blockIdx.x * TX + l;
-  const int o = DIM == 3 ? blockIdx.y : 0;  // y index in 3D
-  const long lstride = DIM == 3 ? (long)g.nx * g.ny : (long)g.nx;
-  const C* tw = DIM == 3 ? tb.twz : tb.twy;
-  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tw[i];
-  for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
-  C* base = phi + (long)o * g.nx + x;
-  C v[MS][Q];
-#pragma unroll
-  for (int m = 0; m < MS; ++m) {
-    C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      C a = base[m * g.npts + (long)(j + P * q) * lstride];
-      v[m][q] = scale ? cmul(a, sc) : a;
-    }
-  }
-  __syncthreads();
-  C* my = lds + l * LS;
-#pragma unroll
-  for (int m = 0; m < MS; ++m) line_fft<N, Q, -1>(v[m], my, lds_tw, j);
-  const T dt = (T)ph.dt;
-  const C kexy = DIM == 3 ? cmul(tb.kea[x], tb.keb[o]) : tb.kea[x];
-  C e1, d[MS];
-  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, d);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int pos = j + P * q;
-    if constexpr (DIM == 2) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[pos], ph, dt, e1, d);
-    C pt[MS];
-#pragma unroll
-    for (int m = 0; m < MS; ++m) pt[m] = cmul(v[m][q], cmul(kexy, lds_ke[m * N + pos]));
-    soc_apply<SPIN, C, T>(pt, e1, d);
-#pragma unroll
-    for (int m = 0; m < MS; ++m) v[m][q] = pt[m];
-  }
-#pragma unroll
-  for (int m = 0; m < MS; ++m) line_fft<N, Q, +1>(v[m], my, lds_tw, j);
-#pragma unroll
-  for (int m = 0; m < MS; ++m)
-#pragma unroll
-    for (int q = 0; q < Q; ++q) base[m * g.npts + (long)(j + P * q) * lstride] = v[m][q];
-}
-
 // ------------------------------------------------ renormalisation logic
 struct FinArgs {
   int imag, soc, dim, ms;
@@ -310,43 +104,283 @@ __device__ inline void renorm_scale(const FinArgs& f, const double* n, double* s
   for (int m = 0; m < f.ms; ++m) c[m] = sigma[m] / tn;
 }
 
-// Sum the NORMC partials in a fixed order, store the norms and the scale.
-// mode 0: step renormalisation; mode 1: plain NORMT (scale 1/TNORM).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, int nparts, FinArgs f,
-                                                   int mode, double* norm_out, C_t<T>* cscale, double* cscale64,
-                                                   double* sig, int* err) {
-  using C = C_t<T>;
-  __shared__ double red[16 * 5];
-  double a[5] = {0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
-    for (int m = 0; m < f.ms; ++m) a[m] += partials[(long)i * f.ms + m];
-  wave_sum<5>(a);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (lane == 0)
-    for (int m = 0; m < 5; ++m) red[wid * 5 + m] = a[m];
+// Where the last block of a pass leaves the step's norms and scale.
+template <typename T> struct FinOut {
+  unsigned int* ticket;  // arrival counter, zero between launches
+  double* norm;          // NORMC of the step [5]
+  C_t<T>* cscale;        // renormalisation per component [5]
+  double* cscale64;
+  double* sig;           // Newton state of the SOC = 0 projection
+  int* err;
+  FinArgs f;
+};
+
+// ----------------------------------------------------------- reductions
+// butterfly sum over the active lanes of a wave; blocks narrower than a wave
+// (small 1D lines) only exchange among their own lanes
+template <int M> __device__ __forceinline__ void wave_sum(double (&a)[M]) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    if (off < (int)blockDim.x)
+#pragma unroll
+      for (int m = 0; m < M; ++m) a[m] += __shfl_xor(a[m], off, 64);
+}
+
+// sum a[M] over the block and write out[blockIdx.x*M + m]; red: >= 16*M doubles
+template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M], double* red, double* out) {
+  wave_sum<M>(a);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double n[5] = {0, 0, 0, 0, 0};
-    for (int w = 0; w < nw; ++w)
-      for (int m = 0; m < f.ms; ++m) n[m] += red[w * 5 + m];
-    double c[5];
-    if (mode == 1) {
-      double t = 0;
-      for (int m = 0; m < f.ms; ++m) t += n[m];
-      t = sqrt(t);
-      for (int m = 0; m < f.ms; ++m) c[m] = 1.0 / t;
-      norm_out[5] = t;
-    } else {
-      renorm_scale(f, n, sig, err, c);
-    }
-    for (int m = 0; m < f.ms; ++m) {
-      norm_out[m] = n[m];
-      cscale64[m] = c[m];
-      cscale[m] = mk<C>((T)c[m], (T)0);
-    }
+  if (lane == 0)
+#pragma unroll
+    for (int m = 0; m < M; ++m) red[wid * M + m] = a[m];
+  __syncthreads();
+  if (threadIdx.x < M) {
+    double s = 0;
+    for (int w = 0; w < nw; ++w) s += red[w * M + threadIdx.x];
+    out[(long)blockIdx.x * M + threadIdx.x] = s;
   }
 }
+
+// The last block to finish a pass sums all blocks' NORMC partials in block
+// order (deterministic), turns them into the step's norms and renormalisation
+// scale, and re-arms the counter.  Release/acquire at device scope makes the
+// other blocks' partials visible across XCDs.
+template <typename T, int MS>
+__device__ __forceinline__ void finalize_last_block(const double* partials, double* red, const FinOut<T>& fo) {
+  __shared__ unsigned int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(fo.ticket, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double a[MS];
+  SFor<0, MS>::run([&](auto mi) { a[decltype(mi)::value] = 0.0; });
+#pragma unroll 8
+  for (unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+    SFor<0, MS>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      a[m] += __builtin_nontemporal_load(&partials[(long)i * MS + m]);
+    });
+  wave_sum<MS>(a);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0)
+    SFor<0, MS>::run([&](auto mi) { red[wid * MS + decltype(mi)::value] = a[decltype(mi)::value]; });
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double n[5] = {0, 0, 0, 0, 0}, c[5];
+    for (int w = 0; w < nw; ++w)
+      for (int m = 0; m < MS; ++m) n[m] += red[w * MS + m];
+    renorm_scale(fo.f, n, fo.sig, fo.err, c);
+    for (int m = 0; m < MS; ++m) {
+      fo.norm[m] = n[m];
+      fo.cscale64[m] = c[m];
+      fo.cscale[m] = mk<C_t<T>>((T)c[m], (T)0);
+    }
+    atomicExch(fo.ticket, 0u);
+  }
+}
+
+// ------------------------------------------------------------ pass A (x)
+// Lines along x, all components.  flags: XF_INV inverse FFT first, XF_PHYS
+// real-space physics (1/N, SE, SP, NORMC partials), XF_FWD forward FFT last,
+// XF_SCALE multiply by the pending scale on load.
+template <typename T, int SPIN, int N, int Q>
+__global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
+                                               double* __restrict__ partials, FinOut<T> fo) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  double* red = reinterpret_cast<double*>(lds_tw + TWS);
+  C* lds = reinterpret_cast<C*>(red + 16 * MS);
+
+  const int lpw = blockDim.x / P;
+  const int tl = threadIdx.x / P, j = threadIdx.x % P;
+  const long line = (long)blockIdx.x * lpw + tl;
+  const long nlines = (long)g.ny * g.nz;
+  const bool valid = line < nlines;
+  const int y = valid ? (int)(line % g.ny) : 0, z = valid ? (int)(line / g.ny) : 0;
+  C* my = lds + tl * LS;
+
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
+
+  C v[MS][Q];
+  const unsigned int voff = (unsigned int)((line * N + j) * (long)sizeof(C));
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * g.npts, g.npts);
+    const C sc = (flags & XF_SCALE) ? tb.cscale[m] : mk<C>((T)1, (T)0);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      const C a = buf.ld(voff, (unsigned int)(P * q * sizeof(C)));
+      v[m][q] = (flags & XF_SCALE) ? cmul(a, sc) : a;
+    });
+  });
+  __syncthreads();
+  if (flags & XF_INV) {
+    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j); });
+  }
+  if (flags & XF_PHYS) {
+    const T dt = (T)ph.dt;
+    const T inv = (T)ph.inv_npts;
+    const T vyz = tb.vy[y] + tb.vz[z];
+    const double wyz = tb.wy[y] * tb.wz[z];
+    double acc[MS];
+    SFor<0, MS>::run([&](auto mi) { acc[decltype(mi)::value] = 0.0; });
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      const int x = j + P * q;
+      C pt[MS];
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        pt[m] = cscale(v[m][q], inv);
+      });
+      if constexpr (SPIN == 1)
+        se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
+      else
+        se2_point<C, T>(pt, ph, dt, ph.dim == 1 ? (T)0.5 : (T)1, sizeof(T) == 4 ? (T)1e-8 : (T)1e-17);
+      sp_point<SPIN, C, T>(pt, tb.vx[x] + vyz, ph, dt);
+      const double wx = tb.wx[x];
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        v[m][q] = pt[m];
+        acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
+      });
+      // one point at a time: keeps the SE temporaries of a single point live
+      __builtin_amdgcn_sched_barrier(0);
+    });
+    SFor<0, MS>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      acc[m] = valid ? acc[m] * wyz : 0.0;
+    });
+    block_sum_store<MS>(acc, red, partials);
+    finalize_last_block<T, MS>(partials, red, fo);
+  }
+  if (flags & XF_FWD) {
+    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
+  }
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * g.npts, g.npts);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      buf.st(v[m][q], voff, (unsigned int)(P * q * sizeof(C)));
+    });
+  });
+}
+
+// ------------------------------------------------------- pass B/D (y)
+// Lines along y for one component; a block owns TX consecutive x at one z.
+// dir -1 forward, +1 inverse; scale applies the pending renormalisation.
+template <typename T, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale) {
+  using C = C_t<T>;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  C* lds = lds_tw + TWS;
+  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int x = blockIdx.x * TX + l;
+  const int z = blockIdx.y, m = blockIdx.z;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twy[i];
+  const CompBuf<C> buf(phi + m * g.npts, g.npts);
+  const unsigned int voff = (unsigned int)(((long)z * g.nx * g.ny + x + (long)j * g.nx) * (long)sizeof(C));
+  const unsigned int sstep = (unsigned int)(P * g.nx * sizeof(C));
+  C v[Q];
+  const C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    const C a = buf.ld(voff, q * sstep);
+    v[q] = scale ? cmul(a, sc) : a;
+  });
+  __syncthreads();
+  if (dir < 0)
+    line_fft<N, Q, -1>(v, lds + l * LS, lds_tw, j);
+  else
+    line_fft<N, Q, +1>(v, lds + l * LS, lds_tw, j);
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    buf.st(v[q], voff, q * sstep);
+  });
+}
+
+// ---------------------------------------------- pass C (k-space lines)
+// Lines along the last axis (z in 3D, y in 2D), all components: forward FFT,
+// KE (SRC/cgpe3d.f90:875-899), SOC (904-1004), inverse FFT.  A block owns TX
+// consecutive x (and one y in 3D).
+template <typename T, int SPIN, int DIM, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  C* lds_ke = lds_tw + TWS;
+  C* lds = lds_ke + MS * N;
+  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int x = blockIdx.x * TX + l;
+  const int o = DIM == 3 ? blockIdx.y : 0;  // y index in 3D
+  const long lstride = DIM == 3 ? (long)g.nx * g.ny : (long)g.nx;
+  const C* tw = DIM == 3 ? tb.twz : tb.twy;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tw[i];
+  for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
+  const unsigned int voff = (unsigned int)(((long)o * g.nx + x + (long)j * lstride) * (long)sizeof(C));
+  const unsigned int sstep = (unsigned int)((long)P * lstride * (long)sizeof(C));
+  C v[MS][Q];
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * g.npts, g.npts);
+    const C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      const C a = buf.ld(voff, q * sstep);
+      v[m][q] = scale ? cmul(a, sc) : a;
+    });
+  });
+  __syncthreads();
+  C* my = lds + l * LS;
+  SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
+  const T dt = (T)ph.dt;
+  const C kexy = DIM == 3 ? cmul(tb.kea[x], tb.keb[o]) : tb.kea[x];
+  C e1, d[MS];
+  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, d);
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    const int pos = j + P * q;
+    if constexpr (DIM == 2) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[pos], ph, dt, e1, d);
+    C pt[MS];
+    SFor<0, MS>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      pt[m] = cmul(v[m][q], cmul(kexy, lds_ke[m * N + pos]));
+    });
+    soc_apply<SPIN, C, T>(pt, e1, d);
+    SFor<0, MS>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      v[m][q] = pt[m];
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j); });
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * g.npts, g.npts);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      buf.st(v[m][q], voff, q * sstep);
+    });
+  });
+}
+
 
 // ------------------------------------------------------------------- 1D
 // One workgroup advances the whole q1D problem nsteps times: forward FFT,
@@ -367,58 +401,71 @@ __global__ __launch_bounds__(N / Q) void k_line1d(C_t<T>* __restrict__ phi, Tabl
   const int j = threadIdx.x;
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
   C v[MS][Q];
-#pragma unroll
-  for (int m = 0; m < MS; ++m)
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[m][q] = phi[m * N + j + P * q];
+  const unsigned int voff = (unsigned int)(j * sizeof(C));
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * N, N);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      v[m][q] = buf.ld(voff, (unsigned int)(P * q * sizeof(C)));
+    });
+  });
   __syncthreads();
   const T dt = (T)ph.dt;
   const T inv = (T)ph.inv_npts;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   for (long it = 0; it < nsteps; ++it) {
-#pragma unroll
-    for (int m = 0; m < MS; ++m) line_fft<N, Q, -1>(v[m], lds, lds_tw, j);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], lds, lds_tw, j); });
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
       const int x = j + P * q;
       C e1, d[MS];
       soc_factors<SPIN, C, T>(tb.kx[x], (T)0, ph, dt, e1, d);
       C pt[MS];
       const C ke = tb.kea[x];
-#pragma unroll
-      for (int m = 0; m < MS; ++m) pt[m] = cmul(v[m][q], ke);
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        pt[m] = cmul(v[m][q], ke);
+      });
       soc_apply<SPIN, C, T>(pt, e1, d);
-#pragma unroll
-      for (int m = 0; m < MS; ++m) v[m][q] = pt[m];
-    }
-#pragma unroll
-    for (int m = 0; m < MS; ++m) line_fft<N, Q, +1>(v[m], lds, lds_tw, j);
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        v[m][q] = pt[m];
+      });
+      __builtin_amdgcn_sched_barrier(0);
+    });
+    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], lds, lds_tw, j); });
     double acc[MS];
-#pragma unroll
-    for (int m = 0; m < MS; ++m) acc[m] = 0.0;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    SFor<0, MS>::run([&](auto mi) { acc[decltype(mi)::value] = 0.0; });
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
       const int x = j + P * q;
       C pt[MS];
-#pragma unroll
-      for (int m = 0; m < MS; ++m) pt[m] = cscale(v[m][q], inv);
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        pt[m] = cscale(v[m][q], inv);
+      });
       if constexpr (SPIN == 1)
         se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
       else
         se2_point<C, T>(pt, ph, dt, (T)0.5, sizeof(T) == 4 ? (T)1e-8 : (T)1e-17);
       sp_point<SPIN, C, T>(pt, tb.vx[x], ph, dt);
       const double wx = tb.wx[x];
-#pragma unroll
-      for (int m = 0; m < MS; ++m) {
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
         v[m][q] = pt[m];
         acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
-      }
-    }
+      });
+      // one point at a time: keeps the SE temporaries of a single point live
+      __builtin_amdgcn_sched_barrier(0);
+    });
     wave_sum<MS>(acc);
     __syncthreads();
     if (lane == 0)
-#pragma unroll
-      for (int m = 0; m < MS; ++m) red[wid * MS + m] = acc[m];
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        red[wid * MS + m] = acc[m];
+      });
     __syncthreads();
     if (threadIdx.x == 0) {
       double n[5] = {0, 0, 0, 0, 0}, c[5];
@@ -432,18 +479,21 @@ __global__ __launch_bounds__(N / Q) void k_line1d(C_t<T>* __restrict__ phi, Tabl
     }
     __syncthreads();
     if (f.imag) {
-#pragma unroll
-      for (int m = 0; m < MS; ++m) {
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
         const T c = (T)bc[m];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) v[m][q] = cscale(v[m][q], c);
-      }
+        SFor<0, Q>::run([&](auto qi) { v[m][decltype(qi)::value] = cscale(v[m][decltype(qi)::value], c); });
+      });
     }
   }
-#pragma unroll
-  for (int m = 0; m < MS; ++m)
-#pragma unroll
-    for (int q = 0; q < Q; ++q) phi[m * N + j + P * q] = v[m][q];
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * N, N);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      buf.st(v[m][q], voff, (unsigned int)(P * q * sizeof(C)));
+    });
+  });
 }
 
 }  // namespace fgpe

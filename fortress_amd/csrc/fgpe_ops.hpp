@@ -13,7 +13,7 @@ template <typename K> void allow_lds(K kernel, size_t bytes) {
 
 // ------------------------------------------------------------- launches
 template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
-  constexpr int Q = Cfg<T>::Q, P = N / Q;
+  constexpr int Q = Cfg<T>::QX, P = N / Q;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const int threads = P >= 256 ? P : 256;
@@ -28,55 +28,66 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
   }
   auto kern = k_xpass<T, SPIN, N, Q>;
   allow_lds(kern, lds);
+  FinOut<T> fo;
+  fo.ticket = h->ticket;
+  fo.norm = h->dnorm;
+  fo.cscale = (C<T>*)h->cscale;
+  fo.cscale64 = h->cscale64;
+  fo.sig = h->dsig;
+  fo.err = h->derr;
+  fo.f = finargs(h);
+  prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
-                     phys(h), flags, h->partials);
+                     phys(h), flags, h->partials, fo);
   HIPCHK(hipGetLastError());
-  if (flags & XF_PHYS) {
-    hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(1024), 0, h->stream, h->partials, blocks, finargs(h), 0,
-                       h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
-    HIPCHK(hipGetLastError());
-  }
+  prof_end(h, (flags & XF_PHYS) ? FGPE_PASS_X : FGPE_PASS_X0);
   return 0;
 }
 
 template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
-  constexpr int Q = Cfg<T>::Q, P = N / Q, TX = Cfg<T>::TX;
+  constexpr int Q = Cfg<T>::QY, P = N / Q, TX = Cfg<T>::TX;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(C<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "y pass needs %zu B of LDS", lds);
   auto kern = k_ypass<T, N, Q, TX>;
   allow_lds(kern, lds);
+  prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(h->nx / TX, h->nz, h->ms), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h),
                      tables<T>(h), dir, scale);
   HIPCHK(hipGetLastError());
+  prof_end(h, FGPE_PASS_Y);
   return 0;
 }
 
 template <typename T, int SPIN, int DIM, int N> int launch_k(fgpe_handle* h, int scale) {
-  constexpr int Q = Cfg<T>::Q, P = N / Q, TX = Cfg<T>::TX;
+  constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TX;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(C<T>) + (size_t)MS * N * sizeof(C<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
   auto kern = k_kpass<T, SPIN, DIM, N, Q, TX>;
   allow_lds(kern, lds);
+  prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(h->nx / TX, DIM == 3 ? h->ny : 1), dim3(TX * P), lds, h->stream, (C<T>*)h->phi,
                      geom(h), tables<T>(h), phys(h), scale);
   HIPCHK(hipGetLastError());
+  prof_end(h, FGPE_PASS_K);
   return 0;
 }
 
 template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps) {
-  constexpr int Q = Cfg<T>::Q, P = N / Q;
+  constexpr int Q = Cfg<T>::Q1, P = N / Q;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(C<T>) + 16 * MS * sizeof(double) + 8 * sizeof(double) + line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "1D step needs %zu B of LDS", lds);
   auto kern = k_line1d<T, SPIN, N, Q>;
   allow_lds(kern, lds);
+  prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(1), dim3(P), lds, h->stream, (C<T>*)h->phi, tables<T>(h), phys(h), finargs(h),
                      nsteps, h->dnorm, h->dsig, h->derr);
   HIPCHK(hipGetLastError());
+  prof_end(h, FGPE_PASS_1D);
   return 0;
 }
 

@@ -35,6 +35,7 @@ EXPORTS = (
     "fgpe_center_abs",
     "fgpe_sync",
     "fgpe_last_run_ms",
+    "fgpe_pass_times",
     "fgpe_last_error",
 )
 
@@ -66,6 +67,7 @@ def lib():
         L.fgpe_conv_max.argtypes = [vp, dp]
         L.fgpe_center_abs.argtypes = [vp, dp]
         L.fgpe_last_run_ms.argtypes = [vp, dp]
+        L.fgpe_pass_times.argtypes = [vp, i64, dp, C.POINTER(C.c_int32)]
         L.fgpe_last_error.argtypes = [C.c_char_p, C.c_size_t]
         for name in EXPORTS:
             if getattr(L, name).restype is not None and name != "fgpe_destroy":
@@ -177,6 +179,16 @@ class Solver:
 
     def sync(self):
         _check(lib().fgpe_sync(self.h), "fgpe_sync")
+
+    PASS_NAMES = ("x", "x0", "y", "k", "finalize", "line1d")
+
+    def pass_times(self, nsteps):
+        """Run nsteps with per-kernel HIP events: {pass: (avg ms, launches)}."""
+        ms = np.zeros(6)
+        cnt = np.zeros(6, dtype=np.int32)
+        _check(lib().fgpe_pass_times(self.h, int(nsteps), _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int32))),
+               "fgpe_pass_times")
+        return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PASS_NAMES) if cnt[i]}
 
     def last_run_ms(self):
         v = C.c_double()

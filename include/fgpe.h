@@ -152,6 +152,21 @@ int fgpe_sync(fgpe_handle* h);
  * only times the whole program, SRC/cgpe3d.f90:218,606-614). */
 int fgpe_last_run_ms(fgpe_handle* h, double* ms);
 
+/* Per-kernel device times of a timed run: advances nsteps steps exactly like
+ * fgpe_run while recording HIP events around every kernel launch on the
+ * handle's stream, and returns in ms_out[FGPE_NPASS] the average duration in
+ * milliseconds of each pass kind (0 where a kind did not run) and in
+ * count_out[FGPE_NPASS] how many launches it averaged.  No reference
+ * counterpart; it backs the per-pass bandwidth report. */
+#define FGPE_PASS_X 0     /* inverse-x + SE + SP + NORMC (+ forward-x) */
+#define FGPE_PASS_X0 1    /* forward-x only (first step of a run) */
+#define FGPE_PASS_Y 2     /* y lines, one component (3D) */
+#define FGPE_PASS_K 3     /* k-space lines with KE + SOC */
+#define FGPE_PASS_FIN 4   /* reserved: NORMC finalisation now runs inside pass X */
+#define FGPE_PASS_1D 5    /* q1D whole-step kernel */
+#define FGPE_NPASS 6
+int fgpe_pass_times(fgpe_handle* h, int64_t nsteps, double* ms_out, int32_t* count_out);
+
 /* Message of the last failing call on this thread. */
 int fgpe_last_error(char* buf, size_t len);
 

