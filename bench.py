@@ -65,6 +65,48 @@ def cpu_baseline(phi0, n, steps):
                       "(oracle/fgpe_oracle.c, OpenMP, %d threads), %.1f s" % (steps, n, threads, dt)}
 
 
+def accuracy(args):
+    """FP32 against FP64 propagation of the same workload from the same state:
+    relative L2 of the component densities |phi_m|^2 at checkpoints."""
+    from fortress_amd import fgpe
+    from fortress_amd.abi import FGPE_FP32, FGPE_FP64
+    p_im, _ = fgpe.derive(workload(args.n, 1, 0.005), FGPE_FP32)
+    with fgpe.Solver(p_im) as s:
+        s.initial_guess()
+        s.normalize()
+        s.run(args.prep_steps)
+        phi_g = s.download()
+    phi0 = noisy(phi_g, seed=1234)
+    del phi_g
+    marks = sorted(set(int(x) for x in args.accuracy.split(",")))
+    p32, _ = fgpe.derive(workload(args.n, 0, 5e-4), FGPE_FP32)
+    p64, _ = fgpe.derive(workload(args.n, 0, 5e-4), FGPE_FP64)
+    s32, s64 = fgpe.Solver(p32), fgpe.Solver(p64)
+    s32.upload(phi0)
+    s64.upload(phi0)
+    del phi0
+    done, rows = 0, []
+    for mark in marks:
+        s32.run(mark - done)
+        s64.run(mark - done)
+        done = mark
+        a = s64.download()
+        num = den = 0.0
+        b = s32.download()
+        for m in range(a.shape[0]):
+            da = np.abs(a[m]) ** 2
+            db = np.abs(b[m]) ** 2
+            num += float(np.sum((db - da) ** 2))
+            den += float(np.sum(da ** 2))
+        phi_err = float(np.sqrt(np.sum(np.abs(b - a) ** 2) / np.sum(np.abs(a) ** 2)))
+        del a, b
+        rows.append({"steps": mark, "density_rel_l2": float(np.sqrt(num / den)), "phi_rel_l2": phi_err})
+        print(json.dumps(rows[-1]), flush=True)
+    s32.close()
+    s64.close()
+    print(json.dumps({"metric": "FP32 vs FP64 component densities, rel L2", "n": args.n, "rows": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,10 +116,15 @@ def main():
     ap.add_argument("--prep-steps", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy", default=None, metavar="STEPS,...",
+                    help="instead of timing, compare FP32 with FP64 propagation at these step counts")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
                  "(the slab decomposition over RCCL is design-only); use --gpus 1")
+
+    if args.accuracy:
+        return accuracy(args)
 
     from fortress_amd import fgpe
     from fortress_amd.abi import FGPE_FP32

@@ -73,8 +73,26 @@ std::vector<double> simpson_weights(int n, double d) {
   return w;
 }
 
-template <typename T> std::vector<C<T>> stage_twiddles(int n, int q) {
-  std::vector<C<T>> t;
+// A unit-modulus constant re + i im in the kernels' rotation format: FP32 as
+// hi + lo parts (their sum is the FP64 value), FP64 as is.
+template <typename T> R_t<T> rot_from(double re, double im);
+template <> R_t<float> rot_from<float>(double re, double im) {
+  R_t<float> w;
+  w.x = (float)re;
+  w.y = (float)im;
+  w.z = (float)(re - (double)w.x);
+  w.w = (float)(im - (double)w.y);
+  return w;
+}
+template <> R_t<double> rot_from<double>(double re, double im) {
+  R_t<double> w;
+  w.x = re;
+  w.y = im;
+  return w;
+}
+
+template <typename T> std::vector<R_t<T>> stage_twiddles(int n, int q) {
+  std::vector<R_t<T>> t;
   int ns = 1;
   while (ns < n) {
     int r = pick_radix(n / ns, q);
@@ -82,28 +100,20 @@ template <typename T> std::vector<C<T>> stage_twiddles(int n, int q) {
       for (int rr = 0; rr < r; ++rr)
         for (int k = 0; k < ns; ++k) {
           double a = -2.0 * PI * (double)(k * rr) / (double)(ns * r);
-          C<T> c;
-          c.x = (T)std::cos(a);
-          c.y = (T)std::sin(a);
-          t.push_back(c);
+          t.push_back(rot_from<T>(std::cos(a), std::sin(a)));
         }
     ns *= r;
   }
-  if (t.empty()) t.push_back(C<T>{(T)1, (T)0});
+  if (t.empty()) t.push_back(rot_from<T>(1.0, 0.0));
   return t;
 }
 
-// exp(-i CDT a) for real a
-template <typename T> C<T> cphase(double a, double dt, int imag) {
-  C<T> c;
-  if (imag) {
-    c.x = (T)std::exp(-dt * a);
-    c.y = (T)0;
-  } else {
-    c.x = (T)std::cos(dt * a);
-    c.y = (T)(-std::sin(dt * a));
-  }
-  return c;
+// exp(-i CDT a) - 1 for real a, formed without cancellation: imaginary time
+// expm1(-DT a), real time -2 sin^2(DT a/2) - i sin(DT a)
+template <typename T> R_t<T> ke_factor(double a, double dt, int imag) {
+  if (imag) return rot_from<T>(std::expm1(-dt * a), 0.0);
+  const double sh = std::sin(0.5 * dt * a);
+  return rot_from<T>(-2.0 * sh * sh, -std::sin(dt * a));
 }
 
 template <typename T> int build_tables(fgpe_handle* h) {
@@ -122,15 +132,16 @@ template <typename T> int build_tables(fgpe_handle* h) {
   if ((rc = upload_vec(h, &h->twz, stage_twiddles<T>(h->nz > 1 ? h->nz : qz, qz)))) return rc;
   const double dt = p.dt;
   const int imag = p.imag;
-  // KE factors: exp(-i CDT kx^2) * exp(-i CDT ky^2) * line factor
-  std::vector<C<T>> kea(h->nx), keb(h->dim == 3 ? h->ny : 1);
-  for (int i = 0; i < h->nx; ++i) kea[i] = cphase<T>(kx[i] * kx[i], dt, imag);
+  // KE factors minus one, per axis (SRC/cgpe3d.f90:875-899; 2D
+  // SRC/cgpe2d.f90:812-829; 1D SRC/cgpe1d.f90:826-842): kx in pass A, ky in
+  // pass B (3D), line axis per component in pass C (3D kz^2 - 2 L GAMMAZ kz,
+  // 2D ky^2)
+  std::vector<R_t<T>> kex(h->nx), key(h->dim == 3 ? h->ny : 1, rot_from<T>(0.0, 0.0));
+  for (int i = 0; i < h->nx; ++i) kex[i] = ke_factor<T>(kx[i] * kx[i], dt, imag);
   if (h->dim == 3)
-    for (int j = 0; j < h->ny; ++j) keb[j] = cphase<T>(ky[j] * ky[j], dt, imag);
-  else
-    keb[0] = C<T>{(T)1, (T)0};
+    for (int j = 0; j < h->ny; ++j) key[j] = ke_factor<T>(ky[j] * ky[j], dt, imag);
   int nl = h->dim == 3 ? h->nz : (h->dim == 2 ? h->ny : 1);
-  std::vector<C<T>> kel((size_t)h->ms * nl);
+  std::vector<R_t<T>> kel((size_t)h->ms * nl);
   for (int m = 0; m < h->ms; ++m) {
     const int L = m - h->spin;
     for (int t = 0; t < nl; ++t) {
@@ -141,11 +152,11 @@ template <typename T> int build_tables(fgpe_handle* h) {
         a = ky[t] * ky[t];
       else
         a = 0.0;
-      kel[(size_t)m * nl + t] = cphase<T>(a, dt, imag);
+      kel[(size_t)m * nl + t] = ke_factor<T>(a, dt, imag);
     }
   }
-  if ((rc = upload_vec(h, &h->kea, kea))) return rc;
-  if ((rc = upload_vec(h, &h->keb, keb))) return rc;
+  if ((rc = upload_vec(h, &h->kex, kex))) return rc;
+  if ((rc = upload_vec(h, &h->key, key))) return rc;
   if ((rc = upload_vec(h, &h->kel, kel))) return rc;
   std::vector<T> kxt(kx.begin(), kx.end()), kyt(ky.begin(), ky.end());
   if ((rc = upload_vec(h, &h->kx, kxt))) return rc;
@@ -381,8 +392,6 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
   if ((rc = dalloc(h, (void**)&h->dnorm, 8 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->dsig, 2 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->derr, sizeof(int)))) return bail(rc);
-  if ((rc = dalloc(h, (void**)&h->ticket, sizeof(unsigned int)))) return bail(rc);
-  if (hipMemset(h->ticket, 0, sizeof(unsigned int)) != hipSuccess) return bail(fail(FGPE_EDEVICE, "hipMemset failed"));
   const double sig0[2] = {20.0, 20.0};  // SRC/cgpe3d.f90:443-444
   if (hipMemcpy(h->dsig, sig0, sizeof sig0, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(FGPE_EDEVICE, "hipMemcpy failed"));

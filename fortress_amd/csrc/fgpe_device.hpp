@@ -55,6 +55,66 @@ template <typename C> __host__ __device__ __forceinline__ C cmuli(C a) { return 
 template <typename C> __host__ __device__ __forceinline__ C cmulmi(C a) { return mk<C>(a.y, -a.x); }
 template <typename C> __host__ __device__ __forceinline__ decltype(C::x) cabs2(C a) { return a.x * a.x + a.y * a.y; }
 
+// ------------------------------------------------- unbiased constant rotations
+// A constant factor w of every step (FFT twiddles, kinetic factors) stored,
+// in FP32, as hi + lo parts that give its FP64 value.  The product is formed
+// by an FMA chain that folds the lo parts in before the last rounding, so the
+// result is rounded once from (nearly) the exact product.  A plain FP32 factor
+// would carry the same rounding of w into every step (|fl(w)| != 1), a bias
+// that accumulates linearly over the run, where unbiased rounding only
+// random-walks.  FP64 keeps a single part.
+template <typename T> struct rot_of;
+template <> struct rot_of<float> { using type = float4; };   // re_hi, im_hi, re_lo, im_lo
+template <> struct rot_of<double> { using type = double2; };  // re, im
+template <typename T> using R_t = typename rot_of<T>::type;
+
+// v * w (DIR < 0) or v * conj(w) (DIR > 0)
+template <int DIR> __host__ __device__ __forceinline__ float2 rot(float2 v, float4 w) {
+  const float wi = DIR > 0 ? -w.y : w.y, wil = DIR > 0 ? -w.w : w.w;
+  const float re = fmaf(v.x, w.x, fmaf(-v.y, wi, fmaf(v.x, w.z, -v.y * wil)));
+  const float im = fmaf(v.x, wi, fmaf(v.y, w.x, fmaf(v.x, wil, v.y * w.z)));
+  return make_float2(re, im);
+}
+template <int DIR> __host__ __device__ __forceinline__ double2 rot(double2 v, double2 w) {
+  return DIR > 0 ? cmulc(v, w) : cmul(v, w);
+}
+// v * (1 + h) = v + v h for a unit-modulus factor near 1 stored as h = w - 1
+// (hi + lo parts in FP32).  The change v h is formed first and added last, so
+// the one rounding at full magnitude happens on a value that varies from step
+// to step.  Applying w itself would round v * fl(cos) with fl(cos) == 1 for
+// small angles: the product is then exact, the lo parts fall below half an ulp
+// and are dropped the same way every step (a norm drift of order angle^2 per
+// step for the slowly rotating low modes that carry the condensate).
+__host__ __device__ __forceinline__ float2 rot1(float2 v, float4 h) {
+  const float dr = fmaf(v.x, h.x, fmaf(-v.y, h.y, fmaf(v.x, h.z, -v.y * h.w)));
+  const float di = fmaf(v.x, h.y, fmaf(v.y, h.x, fmaf(v.x, h.w, v.y * h.z)));
+  return make_float2(v.x + dr, v.y + di);
+}
+__host__ __device__ __forceinline__ double2 rot1(double2 v, double2 h) { return cadd(v, cmul(v, h)); }
+
+// v * (c + i s) for FP64 constants c, s (compile-time after unrolling)
+template <typename C> __host__ __device__ __forceinline__ C rotc(C v, double c, double s) {
+  using T = decltype(C::x);
+  if constexpr (sizeof(T) == 4) {
+    const float ch = (float)c, cl = (float)(c - (double)ch), sh = (float)s, sl = (float)(s - (double)sh);
+    const float re = fmaf(v.x, ch, fmaf(-v.y, sh, fmaf(v.x, cl, -v.y * sl)));
+    const float im = fmaf(v.x, sh, fmaf(v.y, ch, fmaf(v.x, sl, v.y * cl)));
+    return mk<C>(re, im);
+  } else {
+    return mk<C>(v.x * c - v.y * s, v.x * s + v.y * c);
+  }
+}
+// v * a for a real FP64 constant a
+template <typename C> __host__ __device__ __forceinline__ C scalec(C v, double a) {
+  using T = decltype(C::x);
+  if constexpr (sizeof(T) == 4) {
+    const float h = (float)a, l = (float)(a - (double)h);
+    return mk<C>(fmaf(v.x, h, v.x * l), fmaf(v.y, h, v.y * l));
+  } else {
+    return mk<C>(v.x * a, v.y * a);
+  }
+}
+
 // ---------------------------------------------------------------- small DFTs
 // Exact twiddle constants omega_R^e = cos(2 pi e/R) + DIR i sin(2 pi e/R).
 __host__ __device__ constexpr double cos_tab(int e, int R) {
@@ -122,15 +182,12 @@ __host__ __device__ constexpr double sin_tab(int e, int R) {
 
 // v *= omega_R^e (sign DIR), e a compile-time constant after unrolling
 template <int R, int DIR, typename C> __host__ __device__ __forceinline__ C twc(C v, int e) {
-  using T = decltype(C::x);
   e = ((e % R) + R) % R;
   if (e == 0) return v;
   if (4 * e == R) return DIR > 0 ? cmuli(v) : cmulmi(v);
   if (2 * e == R) return mk<C>(-v.x, -v.y);
   if (4 * e == 3 * R) return DIR > 0 ? cmulmi(v) : cmuli(v);
-  const T c = (T)cos_tab(e, R);
-  const T s = (T)(DIR * sin_tab(e, R));
-  return mk<C>(v.x * c - v.y * s, v.x * s + v.y * c);
+  return rotc(v, cos_tab(e, R), DIR * sin_tab(e, R));
 }
 
 template <int R, int DIR, typename C> struct Dft;
@@ -145,14 +202,13 @@ template <int DIR, typename C> struct Dft<2, DIR, C> {
 template <int DIR, typename C> struct Dft<3, DIR, C> {
   __host__ __device__ __forceinline__ static void run(C* v) {
     using T = decltype(C::x);
-    const T h = (T)0.86602540378443864676;
     C s = cadd(v[1], v[2]);
-    C d = csub(v[1], v[2]);
+    C d = scalec(csub(v[1], v[2]), 0.86602540378443864676);  // h d, h = sqrt(3)/2
     C a = v[0];
     v[0] = cadd(a, s);
     C m = mk<C>(a.x - (T)0.5 * s.x, a.y - (T)0.5 * s.y);
     // DIR * i * h * d
-    C t = DIR > 0 ? mk<C>(-h * d.y, h * d.x) : mk<C>(h * d.y, -h * d.x);
+    C t = DIR > 0 ? cmuli(d) : cmulmi(d);
     v[1] = cadd(m, t);
     v[2] = csub(m, t);
   }
@@ -265,7 +321,8 @@ template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
   static constexpr int P = N / Q;
   static constexpr bool LAST = (NS * R == N);
   static_assert(R > 0, "unsupported FFT size / points-per-thread");
-  __device__ __forceinline__ static void run(C (&v)[Q], C* lds, const C* tw, int j) {
+  using Rw = R_t<decltype(C::x)>;
+  __device__ __forceinline__ static void run(C (&v)[Q], C* lds, const Rw* tw, int j) {
     C x[Q];
     SFor<0, Q / R>::run([&](auto ui) {
       constexpr int u = decltype(ui)::value;
@@ -279,9 +336,7 @@ template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
       if constexpr (NS > 1) {
         SFor<1, R>::run([&](auto ri) {
           constexpr int r = decltype(ri)::value;
-          C w = tw[r * NS + k];
-          if constexpr (DIR > 0) w.y = -w.y;
-          y[r] = cmul(y[r], w);
+          y[r] = rot<DIR>(y[r], tw[r * NS + k]);
         });
       }
       Dft<R, DIR, C>::run(y);
@@ -321,7 +376,7 @@ template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
 // of a line; afterwards v[q] = X[j + P*q].  lds: this line's LDS scratch
 // (line_stride(N) elements); tw: the per-stage twiddle tables.
 template <int N, int Q, int DIR, typename C>
-__device__ __forceinline__ void line_fft(C (&v)[Q], C* lds, const C* tw, int j) {
+__device__ __forceinline__ void line_fft(C (&v)[Q], C* lds, const R_t<decltype(C::x)>* tw, int j) {
   FftStage<N, Q, 1, DIR, C>::run(v, lds, tw, j);
 }
 
@@ -340,6 +395,8 @@ __host__ __device__ __forceinline__ void sincos_t(float a, float* s, float* c) {
 __host__ __device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
 __host__ __device__ __forceinline__ float exp_t(float a) { return expf(a); }
 __host__ __device__ __forceinline__ double exp_t(double a) { return exp(a); }
+__host__ __device__ __forceinline__ float expm1_t(float a) { return expm1f(a); }
+__host__ __device__ __forceinline__ double expm1_t(double a) { return expm1(a); }
 
 // exp(-i CDT e) for real e: real time exp(-i DT e), imaginary time exp(-DT e)
 template <typename C, typename T> __host__ __device__ __forceinline__ C phase_of(T e, T dt, int imag) {
@@ -354,50 +411,100 @@ template <typename C, typename T> __host__ __device__ __forceinline__ C mul_micd
   return imag ? mk<C>(-dt * z.x, -dt * z.y) : mk<C>(dt * z.y, -dt * z.x);
 }
 
-// exp(-i CDT theta/DT) for a phase theta = DT*E formed in FP64: the argument
-// is reduced exactly in FP64 (mod 2 pi, or by multiples of ln 2 for the
-// imaginary-time exponential) and only the small remainder is evaluated in
-// the state's precision, so a large DT*E (stiff, dense states) costs no more
-// than one rounding of the result.
-template <typename C> __host__ __device__ __forceinline__ C phase_from64(double theta, int imag) {
-  using T = decltype(C::x);
+// exp(-i CDT E) for a phase theta = DT*E carried as an unevaluated sum
+// hi + lo (|lo| << |hi|): the function is evaluated at hi and corrected to
+// first order in lo, which is exact to rounding since lo is below an ulp of hi.
+template <typename C, typename T> __host__ __device__ __forceinline__ C phase2(T hi, T lo, int imag) {
   if (imag) {
-    const double x = -theta;
-    const double n = rint(x * 1.44269504088896340736);  // 1/ln2
-    const double r = fma(-n, 0.69314718055994530942, x);
-    return mk<C>((T)ldexp((double)exp_t((T)r), (int)n), (T)0);
+    const T e = exp_t(-hi);
+    return mk<C>(e - e * lo, (T)0);
   }
-  const double k = rint(theta * 0.15915494309189533577);  // 1/(2 pi)
-  const double r = fma(-k, 6.28318530717958623200, theta) - k * 2.44929359829470635445e-16;
   T s, c;
-  sincos_t((T)r, &s, &c);
-  return mk<C>(c, -s);
+  sincos_t(hi, &s, &c);
+  // cos(hi+lo) - i sin(hi+lo)
+  return mk<C>(c - lo * s, -(s + lo * c));
 }
 
-// SP: diagonal trap + density phases (SRC/cgpe3d.f90:1023-1063).  The
-// densities, E_m and the phase DT*E_m are formed in FP64 from the stored
-// amplitudes; the multiply stays in the state's precision.
+// hi + lo = a + b exactly (Knuth's two-sum)
+template <typename T> __host__ __device__ __forceinline__ void two_sum(T a, T b, T& hi, T& lo) {
+  hi = a + b;
+  const T bb = hi - a;
+  lo = (a - (hi - bb)) + (b - bb);
+}
+
+// SP: diagonal trap + density phases (SRC/cgpe3d.f90:1023-1063), E_m formed
+// from the stored amplitudes.  In FP32 the phase DT*E_m is carried as a
+// two-float sum: DT*TAU0 as a two-float constant times rho with the product's
+// rounding error recovered by an FMA, and exact two-sums for the trap and
+// spin terms.  A stiff step (DT*TAU0*rho of order 1) then costs no more than
+// the rounding of the inputs, where a plain FP32 product would perturb the
+// coupling TAU0 itself by an ulp in every step.
 template <int SPIN, typename C, typename T>
 __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T dt) {
-  const double t0 = p.tau0, t1 = p.tau1, t2 = p.tau2, dtd = p.dt;
-  (void)dt;
+  constexpr int MS = 2 * SPIN + 1;
+  T a[MS];
+#pragma unroll
+  for (int m = 0; m < MS; ++m) a[m] = ph[m].x * ph[m].x + ph[m].y * ph[m].y;
+  T rho = 0;
+#pragma unroll
+  for (int m = 0; m < MS; ++m) rho += a[m];
+  // per-component spin terms E_m - V - TAU0 rho (SRC/cgpe3d.f90:1030-1060)
+  T es[MS];
   if constexpr (SPIN == 1) {
-    double a[3];
-    for (int m = 0; m < 3; ++m) a[m] = (double)ph[m].x * (double)ph[m].x + (double)ph[m].y * (double)ph[m].y;
-    const double rho = a[0] + a[1] + a[2];
-    const double base = (double)v + t0 * rho;
-    const double e[3] = {base + t1 * (a[0] + a[1] - a[2]), base + t1 * (a[0] + a[2]), base + t1 * (a[2] + a[1] - a[0])};
-    for (int m = 0; m < 3; ++m) ph[m] = cmul(ph[m], phase_from64<C>(dtd * e[m], p.imag));
+    const T t1 = (T)p.tau1;
+    es[0] = t1 * (a[0] + a[1] - a[2]);
+    es[1] = t1 * (a[0] + a[2]);
+    es[2] = t1 * (a[2] + a[1] - a[0]);
   } else {
-    double a[5];
-    for (int m = 0; m < 5; ++m) a[m] = (double)ph[m].x * (double)ph[m].x + (double)ph[m].y * (double)ph[m].y;
-    const double rho = a[0] + a[1] + a[2] + a[3] + a[4];
-    const double fz = 2.0 * a[0] + a[1] - a[3] - 2.0 * a[4];
-    const double base = (double)v + t0 * rho;
-    const double e[5] = {base + 2.0 * t1 * fz + 0.4 * t2 * a[4], base + t1 * fz + 0.4 * t2 * a[3],
-                         base + 0.2 * t2 * a[2], base - t1 * fz + 0.4 * t2 * a[1],
-                         base - 2.0 * t1 * fz + 0.4 * t2 * a[0]};
-    for (int m = 0; m < 5; ++m) ph[m] = cmul(ph[m], phase_from64<C>(dtd * e[m], p.imag));
+    const T t1 = (T)p.tau1, t2 = (T)p.tau2;
+    const T fz = (T)2 * a[0] + a[1] - a[3] - (T)2 * a[4];
+    es[0] = (T)2 * t1 * fz + (T)0.4 * t2 * a[4];
+    es[1] = t1 * fz + (T)0.4 * t2 * a[3];
+    es[2] = (T)0.2 * t2 * a[2];
+    es[3] = -t1 * fz + (T)0.4 * t2 * a[1];
+    es[4] = (T)-2 * t1 * fz + (T)0.4 * t2 * a[0];
+  }
+  if constexpr (sizeof(T) == 8) {
+    const T base = v + (T)p.tau0 * rho;
+#pragma unroll
+    for (int m = 0; m < MS; ++m) ph[m] = cmul(ph[m], phase_of<C, T>(base + es[m], dt, p.imag));
+  } else {
+    // DT*TAU0 = c0h + c0l, DT*V and DT*es_m in FP32
+    const double c0 = p.dt * p.tau0;
+    const T c0h = (T)c0, c0l = (T)(c0 - (double)c0h);
+    const T th = c0h * rho;
+    const T tl = fma(c0h, rho, -th) + c0l * rho;
+    T bh, bl;
+    two_sum(th, dt * v, bh, bl);
+    bl += tl;
+#pragma unroll
+    for (int m = 0; m < MS; ++m) {
+      T hi, lo;
+      two_sum(bh, dt * es[m], hi, lo);
+      lo += bl;
+      if (p.imag) {
+        // exp(-theta) - 1 for small theta (a factor that rounds to 1 would
+        // drop the damping the same way every step), the factor itself where
+        // the damping is strong
+        if (hi > (T)0.25) {
+          ph[m] = cmul(ph[m], phase2<C, T>(hi, lo, 1));
+        } else {
+          const T em1 = expm1_t(-hi);
+          const T d = em1 - lo * ((T)1 + em1);
+          ph[m] = cadd(ph[m], cscale(ph[m], d));
+        }
+      } else {
+        // v + v (exp(-i theta) - 1) with exp(-i theta) - 1 = -2 sin^2(theta/2)
+        // - i sin(theta) (+ the first-order lo term): the phase of a slowly
+        // changing density repeats its rounding from step to step, so the
+        // factor's rounding error must scale with theta, not with 1
+        T s2, c2;
+        sincos_t(hi * (T)0.5, &s2, &c2);
+        const T sn = (T)2 * s2 * c2;
+        const C d = mk<C>((T)-2 * s2 * s2 - lo * sn, -(sn + lo * ((T)1 - (T)2 * s2 * s2)));
+        ph[m] = cadd(ph[m], cmul(ph[m], d));
+      }
+    }
   }
 }
 
@@ -432,7 +539,8 @@ template <typename C> __host__ __device__ __forceinline__ void hmul2(const C* c,
 // SE spin-2 (SRC/cgpe3d.f90:1673-1741): predictor with factor cpred (1/2 in
 // 1D, 1 in 2D/3D), averaged couplings, then exp(-i CDT Hbar) phi.  ZHEEV's
 // V diag(exp(-i CDT w)) V^H is replaced by its Taylor series applied to the
-// vector, summed to rounding (the matrix exponential is unique); when
+// vector, summed to rounding (the matrix exponential is unique) apart from the
+// zeroth term, which is added last so rounding scales with the change; when
 // |CDT|*||Hbar||_inf exceeds 1/2 the exponential is split into equal
 // sub-steps.
 template <typename C, typename T>
@@ -460,7 +568,7 @@ __host__ __device__ __forceinline__ void se2_point(C* ph, const Phys& p, T dt, T
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       term[m] = ph[m];
-      sum[m] = ph[m];
+      sum[m] = mk<C>((T)0, (T)0);
       smax = fmax(smax, fabs(ph[m].x) + fabs(ph[m].y));
     }
     for (int k = 1; k < 40; ++k) {
@@ -476,7 +584,7 @@ __host__ __device__ __forceinline__ void se2_point(C* ph, const Phys& p, T dt, T
       if (tmax <= tol * smax) break;
     }
 #pragma unroll
-    for (int m = 0; m < 5; ++m) ph[m] = sum[m];
+    for (int m = 0; m < 5; ++m) ph[m] = cadd(ph[m], sum[m]);
   }
 }
 
@@ -535,14 +643,16 @@ __host__ __device__ __forceinline__ void se1_point(C* ph, const Phys& p, T dt, T
   ph[2] = cadd(ph[2], t3v);
 }
 
-// SOC factor A(beta) diag(exp(-i CDT L alpha)) A(beta)^H applied to one
-// k-point (SRC/cgpe3d.f90:904-1004).  A(beta) = E A0 with A0 the real
-// constant matrix of the 1D code (SRC/cgpe1d.f90:627-675) and
-// E = diag(e1^S, ..., e1^-S), e1 = exp(-i beta); so the factor is applied as
-// E, A0, D, A0^T, E^H in that order from the right.  d[c] = exp(-i CDT L alpha)
-// for column c = L + S.
+// SOC factor A(beta) diag(exp(-i CDT L alpha)) A(beta)^H at one k-point
+// (SRC/cgpe3d.f90:904-1004).  A(beta) = E A0 with A0 the real constant matrix
+// of the 1D code (SRC/cgpe1d.f90:627-675) and E = diag(e1^S, ..., e1^-S),
+// e1 = exp(-i beta).  Returned is only the change of v,
+//   corr = E A0 diag(dm1) A0^T E^H v,   dm1[c] = exp(-i CDT L alpha) - 1,
+// so that v + corr is the factor applied to v: rounding then scales with the
+// (small, for the low modes that carry the state) change rather than with v,
+// and at alpha = 0 the result is v exactly.
 template <int SPIN, typename C, typename T>
-__host__ __device__ __forceinline__ void soc_apply(C* v, C e1, const C* d) {
+__host__ __device__ __forceinline__ void soc_corr(const C* v, C e1, const C* dm1, C* corr) {
   if constexpr (SPIN == 1) {
     const T s5 = (T)0.70710678118654752440;
     // A0 rows: [.5,-s5,.5],[-s5,0,s5],[.5,s5,.5]
@@ -552,18 +662,16 @@ __host__ __device__ __forceinline__ void soc_apply(C* v, C e1, const C* d) {
     C u0 = cadd(cscale(cadd(w0, w2), (T)0.5), cscale(w1, -s5));
     C u1 = cscale(csub(w2, w0), s5);
     C u2 = cadd(cscale(cadd(w0, w2), (T)0.5), cscale(w1, s5));
-    u0 = cmul(u0, d[0]);
-    u1 = cmul(u1, d[1]);
-    u2 = cmul(u2, d[2]);
+    u0 = cmul(u0, dm1[0]);
+    u1 = cmul(u1, dm1[1]);
+    u2 = cmul(u2, dm1[2]);
     // A0 u
     C r0 = csub(cscale(cadd(u0, u2), (T)0.5), cscale(u1, s5));
     C r1 = cscale(csub(u2, u0), s5);
     C r2 = cadd(cscale(cadd(u0, u2), (T)0.5), cscale(u1, s5));
-    // A0 = [[.5,-s5,.5],[-s5,0,s5],[.5,s5,.5]]:  row0 = .5u0 - s5 u1 + .5u2,
-    // row1 = -s5 u0 + s5 u2, row2 = .5u0 + s5 u1 + .5u2
-    v[0] = cmul(r0, e2);
-    v[1] = cmul(r1, e1);
-    v[2] = r2;
+    corr[0] = cmul(r0, e2);
+    corr[1] = cmul(r1, e1);
+    corr[2] = r2;
   } else {
     const T q = (T)0.61237243569579452455;  // sqrt(0.375)
     C e2 = cmul(e1, e1);
@@ -577,31 +685,32 @@ __host__ __device__ __forceinline__ void soc_apply(C* v, C e1, const C* d) {
     C u2 = cadd(cscale(s04, q), cscale(w2, (T)-0.5));
     C u3 = cadd(cscale(d04, (T)0.5), cscale(d13, (T)0.5));
     C u4 = cadd(cadd(cscale(s04, (T)0.25), cscale(s13, (T)0.5)), cscale(w2, q));
-    u0 = cmul(u0, d[0]);
-    u1 = cmul(u1, d[1]);
-    u2 = cmul(u2, d[2]);
-    u3 = cmul(u3, d[3]);
-    u4 = cmul(u4, d[4]);
+    u0 = cmul(u0, dm1[0]);
+    u1 = cmul(u1, dm1[1]);
+    u2 = cmul(u2, dm1[2]);
+    u3 = cmul(u3, dm1[3]);
+    u4 = cmul(u4, dm1[4]);
     // r = A0 u
     C r0 = cadd(cadd(cscale(u0, (T)0.25), cscale(cadd(u1, u3), (T)-0.5)), cadd(cscale(u2, q), cscale(u4, (T)0.25)));
     C r1 = cscale(cadd(csub(u1, u0), csub(u4, u3)), (T)0.5);
     C r2 = cadd(cscale(cadd(u0, u4), q), cscale(u2, (T)-0.5));
     C r3 = cscale(cadd(csub(u3, u0), csub(u4, u1)), (T)0.5);
     C r4 = cadd(cadd(cscale(u0, (T)0.25), cscale(cadd(u1, u3), (T)0.5)), cadd(cscale(u2, q), cscale(u4, (T)0.25)));
-    v[0] = cmul(r0, e2);
-    v[1] = cmul(r1, e1);
-    v[2] = r2;
-    v[3] = cmulc(r3, e1);
-    v[4] = cmulc(r4, e2);
+    corr[0] = cmul(r0, e2);
+    corr[1] = cmul(r1, e1);
+    corr[2] = r2;
+    corr[3] = cmulc(r3, e1);
+    corr[4] = cmulc(r4, e2);
   }
 }
+
 
 // e1 = exp(-i beta) and d[L+S] = exp(-i CDT L alpha) for one (kx, ky)
 // (ALPHA, BETA as in SRC/cgpe3d.f90:956-961; beta = 0 when alpha <= 1e-40).
 // In 1D the reference uses the constant basis with exponent 2 L GAMMAX kx
 // (SRC/cgpe1d.f90:1518), i.e. e1 = 1 and a signed alpha = 2 GAMMAX kx.
 template <int SPIN, typename C, typename T>
-__host__ __device__ __forceinline__ void soc_factors(T kx, T ky, const Phys& p, T dt, C& e1, C* d) {
+__host__ __device__ __forceinline__ void soc_factors(T kx, T ky, const Phys& p, T dt, C& e1, C* dm1) {
   T alpha;
   if (p.dim == 1) {
     alpha = (T)2 * (T)p.gx * kx;
@@ -615,8 +724,22 @@ __host__ __device__ __forceinline__ void soc_factors(T kx, T ky, const Phys& p, 
     else
       e1 = mk<C>(ax / r, -ay / r);
   }
+  dm1[SPIN] = mk<C>((T)0, (T)0);
 #pragma unroll
-  for (int L = -SPIN; L <= SPIN; ++L) d[L + SPIN] = phase_of<C, T>((T)L * alpha, dt, p.imag);
+  for (int L = 1; L <= SPIN; ++L) {
+    const T th = dt * (T)L * alpha;
+    if (p.imag) {
+      dm1[SPIN + L] = mk<C>(expm1_t(-th), (T)0);
+      dm1[SPIN - L] = mk<C>(expm1_t(th), (T)0);
+    } else {
+      // exp(-i th) - 1 = -2 sin^2(th/2) - i sin(th)
+      T sh, ch;
+      sincos_t(th * (T)0.5, &sh, &ch);
+      const T re = (T)-2 * sh * sh, im = (T)2 * sh * ch;
+      dm1[SPIN + L] = mk<C>(re, -im);
+      dm1[SPIN - L] = mk<C>(re, im);
+    }
+  }
 }
 
 }  // namespace fgpe

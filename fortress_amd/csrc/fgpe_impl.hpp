@@ -64,9 +64,9 @@ struct fgpe_handle {
   void* phi = nullptr;      // [ms][npts] complex T
   void* phi_old = nullptr;  // kept copy for CONV
   void* stage64 = nullptr;  // FP64 staging for upload/download in FP32 mode
-  // tables (T precision unless noted)
+  // tables (T precision unless noted; twiddles and KE factors rot_of<T>)
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;
-  void *kea = nullptr, *keb = nullptr, *kel = nullptr;
+  void *kex = nullptr, *key = nullptr, *kel = nullptr;  // KE factors (rot_of<T>)
   void *kx = nullptr, *ky = nullptr;
   void *vx = nullptr, *vy = nullptr, *vz = nullptr;
   double *wx = nullptr, *wy = nullptr, *wz = nullptr;
@@ -77,7 +77,6 @@ struct fgpe_handle {
   double* dnorm = nullptr;       // [8]: norms, TNORM
   double* dsig = nullptr;        // Newton state
   int* derr = nullptr;
-  unsigned int* ticket = nullptr;  // last-block counter of pass A
   double* partials = nullptr;    // reduction scratch
   size_t partials_n = 0;
   double last_ms = 0;
@@ -93,6 +92,7 @@ struct fgpe_handle {
 
 namespace fgpe_impl {
 template <typename T> using C = fgpe::C_t<T>;
+template <typename T> using R_t = fgpe::R_t<T>;
 
 int dalloc(fgpe_handle* h, void** p, size_t bytes);
 int ensure_partials(fgpe_handle* h, size_t n);
@@ -102,12 +102,12 @@ fgpe::Phys phys(fgpe_handle* h);
 fgpe::FinArgs finargs(fgpe_handle* h);
 template <typename T> inline fgpe::Tables<T> tables(fgpe_handle* h) {
   fgpe::Tables<T> t;
-  t.twx = (const fgpe::C_t<T>*)h->twx;
-  t.twy = (const fgpe::C_t<T>*)h->twy;
-  t.twz = (const fgpe::C_t<T>*)h->twz;
-  t.kea = (const fgpe::C_t<T>*)h->kea;
-  t.keb = (const fgpe::C_t<T>*)h->keb;
-  t.kel = (const fgpe::C_t<T>*)h->kel;
+  t.twx = (const fgpe::R_t<T>*)h->twx;
+  t.twy = (const fgpe::R_t<T>*)h->twy;
+  t.twz = (const fgpe::R_t<T>*)h->twz;
+  t.kex = (const fgpe::R_t<T>*)h->kex;
+  t.key = (const fgpe::R_t<T>*)h->key;
+  t.kel = (const fgpe::R_t<T>*)h->kel;
   t.kx = (const T*)h->kx;
   t.ky = (const T*)h->ky;
   t.vx = (const T*)h->vx;

@@ -17,15 +17,22 @@
 
 namespace fgpe {
 
-enum : int { XF_INV = 1, XF_PHYS = 2, XF_FWD = 4, XF_SCALE = 8 };
+enum : int { XF_INV = 1, XF_PHYS = 2, XF_FWD = 4, XF_SCALE = 8, XF_NORM = 16 };
 
+// KE (SRC/cgpe3d.f90:875-899) is a product of one factor per axis, each
+// diagonal in its wave number, so each factor is applied in the pass that has
+// that axis in k space: kx after the forward x FFT (pass A), ky after the
+// forward y FFT (pass B, 3D), and the line axis with the 3D GAMMAZ term in
+// pass C (2D: ky there).  Each factor w is stored as w - 1 (rot_of format) and
+// applied as v + v (w - 1).
 template <typename T> struct Tables {
-  const C_t<T>* twx;  // per-stage FFT twiddles, x axis
-  const C_t<T>* twy;
-  const C_t<T>* twz;
-  const C_t<T>* kea;  // exp(-i CDT kx^2)            [NX]
-  const C_t<T>* keb;  // exp(-i CDT ky^2) (3D), 1     [NY or 1]
-  const C_t<T>* kel;  // line-axis factor per comp    [MS][N_line]
+  const R_t<T>* twx;  // per-stage FFT twiddles, x axis
+  const R_t<T>* twy;
+  const R_t<T>* twz;
+  const R_t<T>* kex;  // exp(-i CDT kx^2) - 1                          [NX]
+  const R_t<T>* key;  // exp(-i CDT ky^2) - 1 (3D)                     [NY or 1]
+  const R_t<T>* kel;  // line axis: 3D exp(-i CDT (kz^2 - 2 L GAMMAZ kz)) - 1,
+                      // 2D exp(-i CDT ky^2) - 1, per component      [MS][N_line]
   const T* kx;        // wave numbers (FFT order, positive Nyquist)
   const T* ky;
   const T* vx;  // ALPHAX^2 x^2
@@ -104,17 +111,6 @@ __device__ inline void renorm_scale(const FinArgs& f, const double* n, double* s
   for (int m = 0; m < f.ms; ++m) c[m] = sigma[m] / tn;
 }
 
-// Where the last block of a pass leaves the step's norms and scale.
-template <typename T> struct FinOut {
-  unsigned int* ticket;  // arrival counter, zero between launches
-  double* norm;          // NORMC of the step [5]
-  C_t<T>* cscale;        // renormalisation per component [5]
-  double* cscale64;
-  double* sig;           // Newton state of the SOC = 0 projection
-  int* err;
-  FinArgs f;
-};
-
 // ----------------------------------------------------------- reductions
 // butterfly sum over the active lanes of a wave; blocks narrower than a wave
 // (small 1D lines) only exchange among their own lanes
@@ -142,30 +138,23 @@ template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M],
   }
 }
 
-// The last block to finish a pass sums all blocks' NORMC partials in block
-// order (deterministic), turns them into the step's norms and renormalisation
-// scale, and re-arms the counter.  Release/acquire at device scope makes the
-// other blocks' partials visible across XCDs.
+// Sum the NORMC partials of a pass in a fixed order (deterministic), store
+// the norms and the per-component renormalisation scale of the step.
 template <typename T, int MS>
-__device__ __forceinline__ void finalize_last_block(const double* partials, double* red, const FinOut<T>& fo) {
-  __shared__ unsigned int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(fo.ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, int nparts, FinArgs f,
+                                                   double* norm_out, C_t<T>* cscale, double* cscale64, double* sig,
+                                                   int* err) {
+  __shared__ double red[16 * MS];
   double a[MS];
   SFor<0, MS>::run([&](auto mi) { a[decltype(mi)::value] = 0.0; });
-#pragma unroll 8
-  for (unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
     SFor<0, MS>::run([&](auto mi) {
       constexpr int m = decltype(mi)::value;
-      a[m] += __builtin_nontemporal_load(&partials[(long)i * MS + m]);
+      a[m] += partials[(long)i * MS + m];
     });
   wave_sum<MS>(a);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if (lane == 0)
     SFor<0, MS>::run([&](auto mi) { red[wid * MS + decltype(mi)::value] = a[decltype(mi)::value]; });
   __syncthreads();
@@ -173,30 +162,30 @@ __device__ __forceinline__ void finalize_last_block(const double* partials, doub
     double n[5] = {0, 0, 0, 0, 0}, c[5];
     for (int w = 0; w < nw; ++w)
       for (int m = 0; m < MS; ++m) n[m] += red[w * MS + m];
-    renorm_scale(fo.f, n, fo.sig, fo.err, c);
+    renorm_scale(f, n, sig, err, c);
     for (int m = 0; m < MS; ++m) {
-      fo.norm[m] = n[m];
-      fo.cscale64[m] = c[m];
-      fo.cscale[m] = mk<C_t<T>>((T)c[m], (T)0);
+      norm_out[m] = n[m];
+      cscale64[m] = c[m];
+      cscale[m] = mk<C_t<T>>((T)c[m], (T)0);
     }
-    atomicExch(fo.ticket, 0u);
   }
 }
 
 // ------------------------------------------------------------ pass A (x)
 // Lines along x, all components.  flags: XF_INV inverse FFT first, XF_PHYS
 // real-space physics (1/N, SE, SP, NORMC partials), XF_FWD forward FFT last,
-// XF_SCALE multiply by the pending scale on load.
+// XF_SCALE multiply by the pending scale on load, XF_NORM (with XF_PHYS) write
+// the block's NORMC partials.
 template <typename T, int SPIN, int N, int Q>
 __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
-                                               double* __restrict__ partials, FinOut<T> fo) {
+                                               double* __restrict__ partials) {
   using C = C_t<T>;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int P = N / Q;
   constexpr int LS = line_stride(N);
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
   double* red = reinterpret_cast<double*>(lds_tw + TWS);
   C* lds = reinterpret_cast<C*>(red + 16 * MS);
 
@@ -228,7 +217,6 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
   }
   if (flags & XF_PHYS) {
     const T dt = (T)ph.dt;
-    const T inv = (T)ph.inv_npts;
     const T vyz = tb.vy[y] + tb.vz[z];
     const double wyz = tb.wy[y] * tb.wz[z];
     double acc[MS];
@@ -239,7 +227,7 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
       C pt[MS];
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
-        pt[m] = cscale(v[m][q], inv);
+        pt[m] = scalec(v[m][q], ph.inv_npts);
       });
       if constexpr (SPIN == 1)
         se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
@@ -259,11 +247,19 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
       constexpr int m = decltype(mi)::value;
       acc[m] = valid ? acc[m] * wyz : 0.0;
     });
-    block_sum_store<MS>(acc, red, partials);
-    finalize_last_block<T, MS>(partials, red, fo);
+    if (flags & XF_NORM) block_sum_store<MS>(acc, red, partials);
   }
   if (flags & XF_FWD) {
     SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
+    // the kx factor of the next step's KE
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      const R_t<T> f = tb.kex[j + P * q];
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        v[m][q] = rot1(v[m][q], f);
+      });
+    });
   }
   SFor<0, MS>::run([&](auto mi) {
     constexpr int m = decltype(mi)::value;
@@ -277,7 +273,8 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
 
 // ------------------------------------------------------- pass B/D (y)
 // Lines along y for one component; a block owns TX consecutive x at one z.
-// dir -1 forward, +1 inverse; scale applies the pending renormalisation.
+// dir -1 forward (then the ky factor of KE), +1 inverse; scale applies the
+// pending renormalisation.
 template <typename T, int N, int Q, int TX>
 __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale) {
   using C = C_t<T>;
@@ -285,8 +282,8 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi,
   constexpr int LS = line_stride(N);
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
-  C* lds = lds_tw + TWS;
+  R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
+  C* lds = reinterpret_cast<C*>(lds_tw + TWS);
   const int l = threadIdx.x % TX, j = threadIdx.x / TX;
   const int x = blockIdx.x * TX + l;
   const int z = blockIdx.y, m = blockIdx.z;
@@ -302,10 +299,16 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi,
     v[q] = scale ? cmul(a, sc) : a;
   });
   __syncthreads();
-  if (dir < 0)
+  if (dir < 0) {
     line_fft<N, Q, -1>(v, lds + l * LS, lds_tw, j);
-  else
+    // the ky factor of KE
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      v[q] = rot1(v[q], tb.key[j + P * q]);
+    });
+  } else {
     line_fft<N, Q, +1>(v, lds + l * LS, lds_tw, j);
+  }
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;
     buf.st(v[q], voff, q * sstep);
@@ -324,14 +327,14 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
   constexpr int LS = line_stride(N);
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
-  C* lds_ke = lds_tw + TWS;
-  C* lds = lds_ke + MS * N;
+  R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
+  R_t<T>* lds_ke = lds_tw + TWS;
+  C* lds = reinterpret_cast<C*>(lds_ke + MS * N);
   const int l = threadIdx.x % TX, j = threadIdx.x / TX;
   const int x = blockIdx.x * TX + l;
   const int o = DIM == 3 ? blockIdx.y : 0;  // y index in 3D
   const long lstride = DIM == 3 ? (long)g.nx * g.ny : (long)g.nx;
-  const C* tw = DIM == 3 ? tb.twz : tb.twy;
+  const R_t<T>* tw = DIM == 3 ? tb.twz : tb.twy;
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tw[i];
   for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
   const unsigned int voff = (unsigned int)(((long)o * g.nx + x + (long)j * lstride) * (long)sizeof(C));
@@ -351,22 +354,23 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
   C* my = lds + l * LS;
   SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
   const T dt = (T)ph.dt;
-  const C kexy = DIM == 3 ? cmul(tb.kea[x], tb.keb[o]) : tb.kea[x];
-  C e1, d[MS];
-  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, d);
+  C e1, dm1[MS];
+  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, dm1);
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;
     const int pos = j + P * q;
-    if constexpr (DIM == 2) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[pos], ph, dt, e1, d);
-    C pt[MS];
+    if constexpr (DIM == 2) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[pos], ph, dt, e1, dm1);
+    // line-axis KE factor, then the SOC change of the result
+    C pt[MS], cr[MS];
     SFor<0, MS>::run([&](auto mi) {
       constexpr int m = decltype(mi)::value;
-      pt[m] = cmul(v[m][q], cmul(kexy, lds_ke[m * N + pos]));
+      v[m][q] = rot1(v[m][q], lds_ke[m * N + pos]);
+      pt[m] = v[m][q];
     });
-    soc_apply<SPIN, C, T>(pt, e1, d);
+    soc_corr<SPIN, C, T>(pt, e1, dm1, cr);
     SFor<0, MS>::run([&](auto mi) {
       constexpr int m = decltype(mi)::value;
-      v[m][q] = pt[m];
+      v[m][q] = cadd(v[m][q], cr[m]);
     });
     __builtin_amdgcn_sched_barrier(0);
   });
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(N / Q) void k_line1d(C_t<T>* __restrict__ phi, Tabl
   constexpr int P = N / Q;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* lds_tw = reinterpret_cast<C*>(smem_raw);
+  R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
   double* red = reinterpret_cast<double*>(lds_tw + TWS);
   double* bc = red + 16 * MS;  // broadcast slots
   C* lds = reinterpret_cast<C*>(bc + 8);
@@ -412,25 +416,25 @@ __global__ __launch_bounds__(N / Q) void k_line1d(C_t<T>* __restrict__ phi, Tabl
   });
   __syncthreads();
   const T dt = (T)ph.dt;
-  const T inv = (T)ph.inv_npts;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   for (long it = 0; it < nsteps; ++it) {
     SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], lds, lds_tw, j); });
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;
       const int x = j + P * q;
-      C e1, d[MS];
-      soc_factors<SPIN, C, T>(tb.kx[x], (T)0, ph, dt, e1, d);
-      C pt[MS];
-      const C ke = tb.kea[x];
+      C e1, dm1[MS];
+      soc_factors<SPIN, C, T>(tb.kx[x], (T)0, ph, dt, e1, dm1);
+      C pt[MS], cr[MS];
+      const R_t<T> fk = tb.kex[x];
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
-        pt[m] = cmul(v[m][q], ke);
+        v[m][q] = rot1(v[m][q], fk);
+        pt[m] = v[m][q];
       });
-      soc_apply<SPIN, C, T>(pt, e1, d);
+      soc_corr<SPIN, C, T>(pt, e1, dm1, cr);
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
-        v[m][q] = pt[m];
+        v[m][q] = cadd(v[m][q], cr[m]);
       });
       __builtin_amdgcn_sched_barrier(0);
     });
@@ -443,7 +447,7 @@ __global__ __launch_bounds__(N / Q) void k_line1d(C_t<T>* __restrict__ phi, Tabl
       C pt[MS];
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
-        pt[m] = cscale(v[m][q], inv);
+        pt[m] = scalec(v[m][q], ph.inv_npts);
       });
       if constexpr (SPIN == 1)
         se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);

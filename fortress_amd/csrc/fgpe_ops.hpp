@@ -20,7 +20,7 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
   const int lpw = threads / P;
   const long nlines = (long)h->ny * h->nz;
   const int blocks = (int)((nlines + lpw - 1) / lpw);
-  const size_t lds = TWS * sizeof(C<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
+  const size_t lds = TWS * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
   if (flags & XF_PHYS) {
     int rc = ensure_partials(h, (size_t)blocks * MS);
@@ -28,26 +28,25 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
   }
   auto kern = k_xpass<T, SPIN, N, Q>;
   allow_lds(kern, lds);
-  FinOut<T> fo;
-  fo.ticket = h->ticket;
-  fo.norm = h->dnorm;
-  fo.cscale = (C<T>*)h->cscale;
-  fo.cscale64 = h->cscale64;
-  fo.sig = h->dsig;
-  fo.err = h->derr;
-  fo.f = finargs(h);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
-                     phys(h), flags, h->partials, fo);
+                     phys(h), flags, h->partials);
   HIPCHK(hipGetLastError());
   prof_end(h, (flags & XF_PHYS) ? FGPE_PASS_X : FGPE_PASS_X0);
+  if ((flags & XF_PHYS) && (flags & XF_NORM)) {
+    prof_begin(h);
+    hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, blocks,
+                       finargs(h), h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
+    HIPCHK(hipGetLastError());
+    prof_end(h, FGPE_PASS_FIN);
+  }
   return 0;
 }
 
 template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
   constexpr int Q = Cfg<T>::QY, P = N / Q, TX = Cfg<T>::TX;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(C<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
+  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "y pass needs %zu B of LDS", lds);
   auto kern = k_ypass<T, N, Q, TX>;
   allow_lds(kern, lds);
@@ -63,7 +62,7 @@ template <typename T, int SPIN, int DIM, int N> int launch_k(fgpe_handle* h, int
   constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TX;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(C<T>) + (size_t)MS * N * sizeof(C<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
+  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)MS * N * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
   auto kern = k_kpass<T, SPIN, DIM, N, Q, TX>;
   allow_lds(kern, lds);
@@ -79,7 +78,7 @@ template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps
   constexpr int Q = Cfg<T>::Q1, P = N / Q;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(C<T>) + 16 * MS * sizeof(double) + 8 * sizeof(double) + line_stride(N) * sizeof(C<T>);
+  const size_t lds = TWS * sizeof(R_t<T>) + 16 * MS * sizeof(double) + 8 * sizeof(double) + line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "1D step needs %zu B of LDS", lds);
   auto kern = k_line1d<T, SPIN, N, Q>;
   allow_lds(kern, lds);
@@ -158,7 +157,9 @@ template <typename T, int SPIN> int run_steps(fgpe_handle* h, long nsteps) {
     } else {
       if ((rc = dispatch_k<T, SPIN, 2>(h, 1))) return rc;
     }
-    const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0);
+    // NORMC every step in imaginary time (it sets the renormalisation); in
+    // real time the scale is 1 and only the last step's norms are reported
+    const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((h->p.imag || s == nsteps) ? XF_NORM : 0);
     if ((rc = dispatch_x<T, SPIN>(h, flags))) return rc;
   }
   if (h->p.imag) {

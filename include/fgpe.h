@@ -162,7 +162,7 @@ int fgpe_last_run_ms(fgpe_handle* h, double* ms);
 #define FGPE_PASS_X0 1    /* forward-x only (first step of a run) */
 #define FGPE_PASS_Y 2     /* y lines, one component (3D) */
 #define FGPE_PASS_K 3     /* k-space lines with KE + SOC */
-#define FGPE_PASS_FIN 4   /* reserved: NORMC finalisation now runs inside pass X */
+#define FGPE_PASS_FIN 4   /* NORMC finalisation / renormalisation scale */
 #define FGPE_PASS_1D 5    /* q1D whole-step kernel */
 #define FGPE_NPASS 6
 int fgpe_pass_times(fgpe_handle* h, int64_t nsteps, double* ms_out, int32_t* count_out);

