@@ -458,3 +458,55 @@ def test_energy_matches_spectral_expectation(spin, dim):
     assert abs(d_sp) > 1e-3 * abs(e_s)  # the SO coupling matters in this state
     assert abs(d_or - d_sp) < 1e-6 * abs(e_s)
     assert np.allclose(mu - mu0, mu_s - mu0s, atol=1e-6 * np.abs(mu_s).max())
+
+
+# ------------------------------------------------------------ RAD, projection
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rad_gaussian_closed_form(dim):
+    """RAD (SRC/cgpe3d.f90:1224-1268): rms_m = sqrt(int r^2 |phi_m|^2), r^2 the
+    sum over the run's axes, not divided by N_m.  For phi_m = c_m
+    exp(-sum x_d^2 / (2 s_d^2)) the integral is c_m^2 (pi^(d/2) prod s_d)
+    sum_d s_d^2 / 2; anisotropic widths catch a dropped or repeated axis."""
+    n, d = (128, 0.1) if dim == 2 else (64, 0.2)
+    spin = 2
+    inp = case(dim, spin, (n,) * dim, (d,) * dim, 1e-3, (0.0,) * dim, nu=(20.0, 20.0, 20.0))
+    p, _ = O.derive(inp)
+    o = O.Oracle(p)
+    s = (1.0, 0.8, 1.2)[:dim]
+    axes = [(-1.0 + 2.0 * np.arange(n) / n) * n * d / 2.0 for _ in range(dim)]
+    if dim == 2:
+        g = np.exp(-axes[1][:, None] ** 2 / (2 * s[1] ** 2) - axes[0][None, :] ** 2 / (2 * s[0] ** 2))[None]
+    else:
+        g = np.exp(-axes[2][:, None, None] ** 2 / (2 * s[2] ** 2) - axes[1][None, :, None] ** 2 / (2 * s[1] ** 2)
+                   - axes[0][None, None, :] ** 2 / (2 * s[0] ** 2))
+    c = np.array([0.3, 0.5, 1.0, 0.7, 0.2])
+    phi = (c[:, None, None, None] * g[None]).astype(complex)
+    o.set_phi(phi)
+    rms = o.rad()
+    o.close()
+    vol = np.pi ** (dim / 2.0) * np.prod(s)
+    want = np.sqrt(c ** 2 * vol * np.sum(np.square(s)) / 2.0)
+    assert np.allclose(rms, want, rtol=1e-9), (rms, want)
+
+
+@pytest.mark.parametrize("dim,spin,mag", [(1, 1, 0.3), (2, 1, -0.2), (1, 2, 0.5), (3, 2, 0.4)])
+def test_projection_keeps_norm_and_magnetisation(dim, spin, mag):
+    """Without SO coupling the imaginary-time step projects onto total norm 1
+    and magnetisation sum_m m N_m = MAG (SRC/cgpe3d.f90:460-495: sigma for
+    spin 1, NEWTON for spin 2, then NORMT)."""
+    n = {1: (256,), 2: (64, 64), 3: (32, 32, 32)}[dim]
+    d = {1: (0.05,), 2: (0.1, 0.1), 3: (0.4, 0.4, 0.4)}[dim]
+    inp = case(dim, spin, n, d, 5e-4 if spin == 1 else 1.25e-4, (0.0,) * dim, mag=mag,
+               natoms=100000 if dim == 3 else 10000, nu=(20.0, 20.0, 20.0) if dim == 3 else (20.0, 400.0, 400.0),
+               species=NA_SPIN2 if (spin == 2 and dim == 1) else None)
+    p, _ = O.derive(inp)
+    o = O.Oracle(p)
+    o.initial_guess()
+    o.normt()
+    o.run(5)
+    nrm = o.normc()
+    o.close()
+    m = np.arange(spin, -spin - 1, -1)
+    assert abs(nrm.sum() - 1.0) < 1e-12
+    tol = 1e-12 if spin == 1 else 1e-8
+    assert abs(np.dot(m, nrm) - mag) < tol, (np.dot(m, nrm), mag)
