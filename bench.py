@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "3D spin-2 256^3 steps/s (%HBM roofline); time-to-TOL, surrogate vs plain guess"
 PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+# workload (256^3 spin-2 FP32), scaled by the counters' readings for a kernel
+# that reads and writes the spinor once: profiles/passes_3d_spin2_256_hbm_traffic.md
+TRAFFIC_256 = {"x": 1357573606, "y": 1342326214, "k": 1345631389}
 
 
 def workload(n, imag, dt):
@@ -198,7 +202,8 @@ def main():
                    "grid": [args.n] * 3, "components": 5, "stp": stp, "prep": {"imag_steps": args.prep_steps,
                                                                               "dt": 0.005, "energy": prep_en}},
         "roofline": {"bound": "hbm", "kernel": kernel_names[dom], "achieved": round(achieved, 1),
-                     "peak": PEAK_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
+                     "peak": PEAK_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
+                     "traffic": TRAFFIC_256.get(dom) if args.n == 256 else None,
                      "bytes_per_launch": b_pass},
         "step_roofline": {"bytes_per_step": b_alg, "achieved_GBps": round(b_alg * steps_per_s / 1e9, 1),
                           "frac": round(b_alg * steps_per_s / 1e9 / PEAK_GBS, 4)},

@@ -477,12 +477,12 @@ __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T d
     T bh, bl;
     two_sum(th, dt * v, bh, bl);
     bl += tl;
+    if (p.imag) {
 #pragma unroll
-    for (int m = 0; m < MS; ++m) {
-      T hi, lo;
-      two_sum(bh, dt * es[m], hi, lo);
-      lo += bl;
-      if (p.imag) {
+      for (int m = 0; m < MS; ++m) {
+        T hi, lo;
+        two_sum(bh, dt * es[m], hi, lo);
+        lo += bl;
         // exp(-theta) - 1 for small theta (a factor that rounds to 1 would
         // drop the damping the same way every step), the factor itself where
         // the damping is strong
@@ -493,15 +493,35 @@ __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T d
           const T d = em1 - lo * ((T)1 + em1);
           ph[m] = cadd(ph[m], cscale(ph[m], d));
         }
-      } else {
-        // v + v (exp(-i theta) - 1) with exp(-i theta) - 1 = -2 sin^2(theta/2)
-        // - i sin(theta) (+ the first-order lo term): the phase of a slowly
-        // changing density repeats its rounding from step to step, so the
-        // factor's rounding error must scale with theta, not with 1
-        T s2, c2;
-        sincos_t(hi * (T)0.5, &s2, &c2);
-        const T sn = (T)2 * s2 * c2;
-        const C d = mk<C>((T)-2 * s2 * s2 - lo * sn, -(sn + lo * ((T)1 - (T)2 * s2 * s2)));
+      }
+    } else {
+      // v + v (exp(-i theta_m) - 1): the phase of a slowly changing density
+      // repeats its rounding from step to step, so the factor's rounding error
+      // must scale with theta, not with 1.  theta_m = theta_b + delta_m with
+      // the common part theta_b = DT (V + TAU0 rho) (two-float) and the small
+      // spin part delta_m = DT es_m:
+      //   exp(-i theta_m) - 1 = db + dm + db dm,
+      // db from one sincos of theta_b/2 (-2 sin^2 - i sin, plus the lo term),
+      // dm from short series when |delta_m| <= 1/4 (else one more sincos).
+      T s2, c2;
+      sincos_t(bh * (T)0.5, &s2, &c2);
+      const T sn = (T)2 * s2 * c2, cm = (T)-2 * s2 * s2;
+      const C db = mk<C>(cm - bl * sn, -(sn + bl * ((T)1 + cm)));
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const T dl = dt * es[m];
+        C dm;
+        if (fabs(dl) <= (T)0.25) {
+          const T q = dl * dl;
+          const T sd = dl * ((T)1 - q * ((T)1 / 6) * ((T)1 - q * ((T)1 / 20) * ((T)1 - q * ((T)1 / 42))));
+          const T cd = -q * (T)0.5 * ((T)1 - q * ((T)1 / 12) * ((T)1 - q * ((T)1 / 30) * ((T)1 - q * ((T)1 / 56))));
+          dm = mk<C>(cd, -sd);
+        } else {
+          T s3, c3;
+          sincos_t(dl * (T)0.5, &s3, &c3);
+          dm = mk<C>((T)-2 * s3 * s3, (T)-2 * s3 * c3);
+        }
+        const C d = cadd(cadd(db, dm), cmul(db, dm));
         ph[m] = cadd(ph[m], cmul(ph[m], d));
       }
     }
