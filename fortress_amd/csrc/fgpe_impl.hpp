@@ -49,6 +49,87 @@ template <> struct Cfg<double> {
   static constexpr int TX = 8;
 };
 
+// ---------------------------------------------------- host-side tables
+// wave numbers in FFT order with positive Nyquist (SRC/cgpe3d.f90:702-726)
+inline std::vector<double> kgrid(int n, double d) {
+  std::vector<double> k(n, 0.0);
+  if (n == 1) return k;
+  double l = n * d;
+  for (int i = 1; i <= 1 + n / 2; ++i) k[i - 1] = (i - 1.0) * 2.0 * PI / l;
+  for (int i = 1; i <= n / 2 - 1; ++i) k[i + n / 2] = -k[n / 2 - i];
+  return k;
+}
+// x_i = (-1 + 2(i-1)/N) L/2 (SRC/cgpe3d.f90:729-744)
+inline std::vector<double> xgrid(int n, double d) {
+  std::vector<double> x(n, 0.0);
+  if (n == 1) return x;
+  double l = n * d;
+  for (int i = 1; i <= n; ++i) x[i - 1] = (-1.0 + 2.0 * (i - 1.0) / n) * l / 2.0;
+  return x;
+}
+// Simpson weights of SRC/simpson.f90:4-22 as a vector: sum_i w_i f_i equals
+// SIMPSON(f, d) for every f.
+inline std::vector<double> simpson_weights(int n, double d) {
+  std::vector<double> w(n, 0.0);
+  if (n == 1) {
+    w[0] = 1.0;
+    return w;
+  }
+  // 1-based: w(1)=d/3; even 2..N-4: 4d/3; odd 3..N-5: 2d/3; w(N-3)=d/3+3d/8;
+  // w(N-2)=w(N-1)=9d/8; w(N)=3d/8
+  w[0] = d / 3.0;
+  for (int i = 2; i <= n - 4; i += 2) w[i - 1] = 4.0 * d / 3.0;
+  for (int i = 3; i <= n - 5; i += 2) w[i - 1] = 2.0 * d / 3.0;
+  w[n - 4] = d / 3.0 + 3.0 * d / 8.0;
+  w[n - 3] = 9.0 * d / 8.0;
+  w[n - 2] = 9.0 * d / 8.0;
+  w[n - 1] = 3.0 * d / 8.0;
+  return w;
+}
+
+// A unit-modulus constant re + i im in the kernels' rotation format: FP32 as
+// hi + lo parts (their sum is the FP64 value), FP64 as is.
+template <typename T> R_t<T> rot_from(double re, double im);
+template <> inline R_t<float> rot_from<float>(double re, double im) {
+  R_t<float> w;
+  w.x = (float)re;
+  w.y = (float)im;
+  w.z = (float)(re - (double)w.x);
+  w.w = (float)(im - (double)w.y);
+  return w;
+}
+template <> inline R_t<double> rot_from<double>(double re, double im) {
+  R_t<double> w;
+  w.x = re;
+  w.y = im;
+  return w;
+}
+
+template <typename T> std::vector<R_t<T>> stage_twiddles(int n, int q) {
+  std::vector<R_t<T>> t;
+  int ns = 1;
+  while (ns < n) {
+    int r = pick_radix(n / ns, q);
+    if (ns > 1)
+      for (int rr = 0; rr < r; ++rr)
+        for (int k = 0; k < ns; ++k) {
+          double a = -2.0 * PI * (double)(k * rr) / (double)(ns * r);
+          t.push_back(rot_from<T>(std::cos(a), std::sin(a)));
+        }
+    ns *= r;
+  }
+  if (t.empty()) t.push_back(rot_from<T>(1.0, 0.0));
+  return t;
+}
+
+// exp(-i CDT a) - 1 for real a, formed without cancellation: imaginary time
+// expm1(-DT a), real time -2 sin^2(DT a/2) - i sin(DT a)
+template <typename T> R_t<T> ke_factor(double a, double dt, int imag) {
+  if (imag) return rot_from<T>(std::expm1(-dt * a), 0.0);
+  const double sh = std::sin(0.5 * dt * a);
+  return rot_from<T>(-2.0 * sh * sh, -std::sin(dt * a));
+}
+
 }  // namespace fgpe_impl
 
 struct fgpe_handle {
@@ -92,7 +173,6 @@ struct fgpe_handle {
 
 namespace fgpe_impl {
 template <typename T> using C = fgpe::C_t<T>;
-template <typename T> using R_t = fgpe::R_t<T>;
 
 int dalloc(fgpe_handle* h, void** p, size_t bytes);
 int ensure_partials(fgpe_handle* h, size_t n);

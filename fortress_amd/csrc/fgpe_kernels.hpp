@@ -56,7 +56,7 @@ struct FinArgs {
 };
 
 // NEWTON/USRFUN (SRC/cgpe3d.f90:1516-1588) on the device
-__device__ inline int newton_dev(double* x, const double* n, double mag, double dt) {
+__device__ __forceinline__ int newton_dev(double* x, const double* n, double mag, double dt) {
   double p0 = 0, p1 = 0;
   for (int k = 0; k < 1000; ++k) {
     double e1 = exp(2.0 * (x[0] + 2.0 * x[1]) * dt), e2 = exp(2.0 * (x[0] + x[1]) * dt), e3 = exp(2.0 * x[0] * dt),
@@ -80,7 +80,7 @@ __device__ inline int newton_dev(double* x, const double* n, double mag, double 
 // norms n[ms] -> per-component scale c[ms] applied after the step:
 // real time 1; imaginary time sigma_m / TNORM with sigma from the SOC = 0
 // projection (SRC/cgpe3d.f90:460-491) or 1 with SO coupling (493).
-__device__ inline void renorm_scale(const FinArgs& f, const double* n, double* sig, int* err, double* c) {
+__device__ __forceinline__ void renorm_scale(const FinArgs& f, const double* n, double* sig, int* err, double* c) {
   const double EPS = 1.0e-40;
   if (!f.imag) {
     for (int m = 0; m < f.ms; ++m) c[m] = 1.0;

@@ -51,19 +51,19 @@ struct ObsArgs {
 // Columns of ENERGY (3D SRC/cgpe3d.f90:1381-1503, 2D SRC/cgpe2d.f90:1145-1267,
 // 1D SRC/cgpe1d.f90:1150-1233) at one point, then RAD's r^2|phi|^2 and NORMC's
 // |phi|^2, all weighted by the separable Simpson weight and summed.
+// One point q of the grid g: adds its Simpson-weighted contributions to
+// acc[3 MS + 2] (ENERGY's EN and MZ integrands, the MU numerators, RAD's
+// r^2 |phi_m|^2 and NORMC's |phi_m|^2).
 template <typename T, int SPIN, int DIM>
-__global__ __launch_bounds__(256) void k_observables(const C_t<T>* __restrict__ phi, Geom g, ObsArgs a,
-                                                     double* __restrict__ partials) {
+__device__ __forceinline__ void obs_point(const C_t<T>* __restrict__ phi, const Geom& g, const ObsArgs& a, long q,
+                                          double* acc) {
   constexpr int MS = 2 * SPIN + 1;
-  constexpr int NC = 3 * MS + 2;
-  __shared__ double red[16 * NC];
-  double acc[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   const double s2 = 1.41421356237309504880, s6 = 2.44948974278317809820, s15 = 1.22474487139158904910;
   const double t0 = a.tau0, t1 = a.tau1, t2 = a.tau2, gx = a.gx, gy = a.gy, gz = a.gz;
   const Z IM(0.0, 1.0);
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < g.npts; q += (long)gridDim.x * blockDim.x) {
+  (void)s6;
+  (void)gz;
+  {
     const int i = (int)(q % g.nx);
     const int jy = (int)((q / g.nx) % g.ny);
     const int k = (int)(q / ((long)g.nx * g.ny));
@@ -232,6 +232,19 @@ __global__ __launch_bounds__(256) void k_observables(const C_t<T>* __restrict__ 
       acc[2 * MS + 2 + m] += w * am[m];
     }
   }
+}
+
+template <typename T, int SPIN, int DIM>
+__global__ __launch_bounds__(256) void k_observables(const C_t<T>* __restrict__ phi, Geom g, ObsArgs a,
+                                                     double* __restrict__ partials) {
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int NC = 3 * MS + 2;
+  __shared__ double red[16 * NC];
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < g.npts; q += (long)gridDim.x * blockDim.x)
+    obs_point<T, SPIN, DIM>(phi, g, a, q, acc);
   block_sum_store<NC>(acc, red, partials);
 }
 

@@ -37,7 +37,26 @@ EXPORTS = (
     "fgpe_last_run_ms",
     "fgpe_pass_times",
     "fgpe_last_error",
+    "fgpe_batch_create",
+    "fgpe_batch_destroy",
+    "fgpe_batch_initial_guess",
+    "fgpe_batch_upload",
+    "fgpe_batch_download",
+    "fgpe_batch_run",
+    "fgpe_batch_solve",
+    "fgpe_batch_result_get",
+    "fgpe_batch_tune",
+    "fgpe_batch_last_ms",
 )
+
+BATCH_NSCHED = 8
+BATCH_SCHEDULES = ((64, 8), (64, 4), (128, 8), (128, 16), (192, 12), (192, 6), (256, 8), (256, 16))
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("done", C.c_int32), ("err", C.c_int32), ("en", C.c_double),
+                ("mz", C.c_double), ("conv", C.c_double), ("conv_en", C.c_double), ("maxdiff", C.c_double),
+                ("mu", C.c_double * 5), ("rms", C.c_double * 5), ("norm", C.c_double * 5)]
 
 
 class FgpeError(RuntimeError):
@@ -69,8 +88,19 @@ def lib():
         L.fgpe_last_run_ms.argtypes = [vp, dp]
         L.fgpe_pass_times.argtypes = [vp, i64, dp, C.POINTER(C.c_int32)]
         L.fgpe_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        L.fgpe_batch_create.argtypes = [C.POINTER(FgpeParams), i32, C.POINTER(vp)]
+        L.fgpe_batch_destroy.argtypes = [vp]
+        L.fgpe_batch_destroy.restype = None
+        L.fgpe_batch_initial_guess.argtypes = [vp]
+        L.fgpe_batch_upload.argtypes = [vp, i32, dp]
+        L.fgpe_batch_download.argtypes = [vp, i32, dp]
+        L.fgpe_batch_run.argtypes = [vp, i64]
+        L.fgpe_batch_solve.argtypes = [vp, i32, C.c_double, i64, C.POINTER(C.c_int32)]
+        L.fgpe_batch_result_get.argtypes = [vp, i32, C.POINTER(BatchResult)]
+        L.fgpe_batch_tune.argtypes = [vp, i32, dp, C.POINTER(C.c_int32)]
+        L.fgpe_batch_last_ms.argtypes = [vp, dp]
         for name in EXPORTS:
-            if getattr(L, name).restype is not None and name != "fgpe_destroy":
+            if getattr(L, name).restype is not None and name not in ("fgpe_destroy", "fgpe_batch_destroy"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -193,4 +223,73 @@ class Solver:
     def last_run_ms(self):
         v = C.c_double()
         _check(lib().fgpe_last_run_ms(self.h, C.byref(v)), "fgpe_last_run_ms")
+        return v.value
+
+
+class Batch:
+    """Independent q2D spin-1 instances advanced together (fgpe_batch_*)."""
+
+    def __init__(self, params_list):
+        arr = (FgpeParams * len(params_list))(*params_list)
+        self.params = list(params_list)
+        self.count = len(params_list)
+        b = C.c_void_p()
+        _check(lib().fgpe_batch_create(arr, self.count, C.byref(b)), "fgpe_batch_create")
+        self.h = b
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgpe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def shape(self, i):
+        p = self.params[i]
+        return (3, 1, p.n[1], p.n[0])
+
+    def initial_guess(self):
+        _check(lib().fgpe_batch_initial_guess(self.h), "fgpe_batch_initial_guess")
+
+    def upload(self, i, phi):
+        a = np.ascontiguousarray(np.asarray(phi, dtype=np.complex128).reshape(self.shape(i)))
+        _check(lib().fgpe_batch_upload(self.h, int(i), _dp(a.view(np.float64))), "fgpe_batch_upload")
+
+    def download(self, i):
+        a = np.empty(self.shape(i), dtype=np.complex128)
+        _check(lib().fgpe_batch_download(self.h, int(i), _dp(a.view(np.float64))), "fgpe_batch_download")
+        return a
+
+    def run(self, nsteps):
+        _check(lib().fgpe_batch_run(self.h, int(nsteps)), "fgpe_batch_run")
+
+    def solve(self, stp, tol, max_steps):
+        n = C.c_int32()
+        _check(lib().fgpe_batch_solve(self.h, int(stp), float(tol), int(max_steps), C.byref(n)), "fgpe_batch_solve")
+        return n.value
+
+    def result(self, i):
+        r = BatchResult()
+        _check(lib().fgpe_batch_result_get(self.h, int(i), C.byref(r)), "fgpe_batch_result_get")
+        return r
+
+    def tune(self, trial_steps):
+        ms = np.zeros(BATCH_NSCHED)
+        chosen = (C.c_int32 * 4)()
+        _check(lib().fgpe_batch_tune(self.h, int(trial_steps), _dp(ms), chosen), "fgpe_batch_tune")
+        return ms, list(chosen)
+
+    def last_ms(self):
+        v = C.c_double()
+        _check(lib().fgpe_batch_last_ms(self.h, C.byref(v)), "fgpe_batch_last_ms")
         return v.value

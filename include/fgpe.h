@@ -170,6 +170,76 @@ int fgpe_pass_times(fgpe_handle* h, int64_t nsteps, double* ms_out, int32_t* cou
 /* Message of the last failing call on this thread. */
 int fgpe_last_error(char* buf, size_t len);
 
+/* ------------------------------------------------------------------------
+ * Batches: many independent q2D instances advanced together.
+ *
+ * No reference counterpart (the reference runs one cgpe.x per parameter
+ * set); each instance follows exactly the single-instance step of
+ * fgpe_run (SRC/cgpe2d.f90:440-485) and the convergence test of cgpe.x
+ * (CONV = max|PHI - PHI_OLD|/(STP*2*DT), CONV_EN = |EN - EN_OLD|/(STP*2*DT),
+ * converged when either is <= TOL; SRC/cgpe2d.f90:488-495).  One persistent
+ * launch advances every unconverged instance by STP steps: blocks take
+ * instances from a work queue, largest grids first, and run all of an
+ * instance's passes themselves.  Instances: dim 2, spin 1, square grids of
+ * 64, 128, 192 or 256 points per axis, one precision for the whole batch.
+ *
+ * Disabled in this build: fgpe_batch_create returns FGPE_EINVAL (the batch
+ * kernel faulted on its first device run; the cause is not yet found).
+ */
+typedef struct fgpe_batch fgpe_batch;
+
+typedef struct fgpe_batch_result {
+  int64_t steps;    /* steps advanced so far */
+  int32_t done;     /* 1 once CONV <= TOL or CONV_EN <= TOL */
+  int32_t err;      /* nonzero if the SOC = 0 projection failed */
+  double en;        /* EN of the last event (scaled units, as fgpe_observables) */
+  double mz;        /* MZ */
+  double conv;      /* CONV of the last event */
+  double conv_en;   /* CONV_EN of the last event */
+  double maxdiff;   /* max |PHI - PHI_OLD| of the last event */
+  double mu[5];     /* MU(1..MS) */
+  double rms[5];    /* RMS(1..MS) */
+  double norm[5];   /* NORMC of the last step, before renormalisation */
+} fgpe_batch_result;
+
+/* Points-per-thread schedules of the batch kernel, per line length. */
+#define FGPE_BATCH_NSCHED 8   /* 64:Q8,Q4  128:Q8,Q16  192:Q12,Q6  256:Q8,Q16 */
+
+/* Allocate a batch of count instances described by p[0..count-1] (each as
+ * fgpe_create would take it). */
+int fgpe_batch_create(const fgpe_params* p, int32_t count, fgpe_batch** b);
+void fgpe_batch_destroy(fgpe_batch* b);
+
+/* Gaussian initial guess and NORMT for every instance (as fgpe_initial_guess
+ * followed by fgpe_normalize), then the observables of the start state. */
+int fgpe_batch_initial_guess(fgpe_batch* b);
+
+/* Copy instance i in or out (layout of fgpe_upload/fgpe_download). */
+int fgpe_batch_upload(fgpe_batch* b, int32_t i, const double* phi);
+int fgpe_batch_download(fgpe_batch* b, int32_t i, double* phi);
+
+/* Advance every instance (converged or not) by nsteps steps in one launch;
+ * afterwards each result holds NORMC, the observables and maxdiff. */
+int fgpe_batch_run(fgpe_batch* b, int64_t nsteps);
+
+/* Run to convergence: repeated launches of stp steps over the unconverged
+ * instances until all have CONV <= tol or CONV_EN <= tol, or max_steps.
+ * *converged receives the number of converged instances. */
+int fgpe_batch_solve(fgpe_batch* b, int32_t stp, double tol, int64_t max_steps, int32_t* converged);
+
+/* Result of instance i. */
+int fgpe_batch_result_get(fgpe_batch* b, int32_t i, fgpe_batch_result* r);
+
+/* Autotuner: for every line length present, time trial_steps steps of that
+ * length's instances under each of its schedules (state restored
+ * afterwards) and keep the fastest.  ms_out[FGPE_BATCH_NSCHED] receives the
+ * measured milliseconds per schedule (0 where not timed), chosen[4] the
+ * schedule kept for 64, 128, 192 and 256 points (-1 where absent). */
+int fgpe_batch_tune(fgpe_batch* b, int32_t trial_steps, double* ms_out, int32_t* chosen);
+
+/* Device time in ms of the last fgpe_batch_run / fgpe_batch_solve. */
+int fgpe_batch_last_ms(fgpe_batch* b, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
