@@ -47,6 +47,16 @@ EXPORTS = (
     "fgpe_batch_result_get",
     "fgpe_batch_tune",
     "fgpe_batch_last_ms",
+    "fgpe_mlp_create",
+    "fgpe_mlp_destroy",
+    "fgpe_mlp_param_count",
+    "fgpe_mlp_get_params",
+    "fgpe_mlp_set_params",
+    "fgpe_mlp_forward",
+    "fgpe_mlp_gradients",
+    "fgpe_mlp_set_dataset",
+    "fgpe_mlp_train_step",
+    "fgpe_mlp_gemm_test",
 )
 
 BATCH_NSCHED = 8

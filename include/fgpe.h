@@ -240,6 +240,37 @@ int fgpe_batch_tune(fgpe_batch* b, int32_t trial_steps, double* ms_out, int32_t*
 /* Device time in ms of the last fgpe_batch_run / fgpe_batch_solve. */
 int fgpe_batch_last_ms(fgpe_batch* b, double* ms);
 
+/* ------------------------------------------------------------------------
+ * Initial-guess surrogate: a fully connected network y = act(x W + b)
+ * (ReLU between layers, none after the last), W stored [K][N] row major,
+ * trained with mean-squared error and Adam.  Products run on the matrix
+ * cores in BF16 with FP32 accumulation; master weights, biases and the loss
+ * stay FP32.  No reference counterpart: the reference starts every run from
+ * its Gaussian or Thomas-Fermi guess (SRC/cgpe2d.f90:697-771).  Parameters
+ * are packed layer by layer as W (K*N floats) then b (N floats).
+ */
+typedef struct fgpe_mlp fgpe_mlp;
+
+/* dims[0..nlayers]: input width, hidden widths, output width. */
+int fgpe_mlp_create(const int32_t* dims, int32_t nlayers, int32_t max_batch, uint64_t seed, fgpe_mlp** m);
+void fgpe_mlp_destroy(fgpe_mlp* m);
+int fgpe_mlp_param_count(fgpe_mlp* m, int64_t* n);
+int fgpe_mlp_get_params(fgpe_mlp* m, float* params);
+int fgpe_mlp_set_params(fgpe_mlp* m, const float* params);
+/* y[batch][out] = network(x[batch][in]) (y may be NULL). */
+int fgpe_mlp_forward(fgpe_mlp* m, const float* x, int32_t batch, float* y);
+/* Gradients of the mean-squared error against t[batch][out], packed like the
+ * parameters (grad may be NULL), and the loss. */
+int fgpe_mlp_gradients(fgpe_mlp* m, const float* x, const float* t, int32_t batch, float* grad, double* loss);
+/* Keep a training set on the device: x[rows][in], y[rows][out]. */
+int fgpe_mlp_set_dataset(fgpe_mlp* m, const float* x, const float* y, int32_t rows);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on dataset rows idx[0..batch). */
+int fgpe_mlp_train_step(fgpe_mlp* m, const int32_t* idx, int32_t batch, float lr, double* loss);
+/* The matrix-core product alone: C[M][N] = op(A) op(B) for BF16 bit patterns
+ * (ta: A given as [K][M]; tb: B given as [N][K]). */
+int fgpe_mlp_gemm_test(int32_t ta, int32_t tb, const uint16_t* a, const uint16_t* b, float* c, int32_t m, int32_t n,
+                       int32_t k);
+
 #ifdef __cplusplus
 }
 #endif
