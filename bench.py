@@ -111,6 +111,66 @@ def accuracy(args):
     print(json.dumps({"metric": "FP32 vs FP64 component densities, rel L2", "n": args.n, "rows": rows}), flush=True)
 
 
+def surrogate_bench(args):
+    """Learned initial guess against the Gaussian guess: ground-state
+    time-to-TOL of q2D spin-1 instances on the production grid, held-out
+    parameter draws, same solver and convergence test for both."""
+    from fortress_amd import surrogate as S
+    rng = np.random.default_rng(args.seed)
+    train = [S.draw(rng) for _ in range(args.train)]
+    held = [S.draw(rng) for _ in range(args.held_out)]
+    t0 = time.perf_counter()
+    feats, targets = [], []
+    for i, d in enumerate(train):
+        phi, k, _, t = S.solve(S.problem(d, S.COARSE), tol=args.train_tol, max_steps=args.max_steps)
+        feats.append(S.features(d))
+        targets.append(S.to_vector(phi))
+        print(json.dumps({"train_state": i, "steps": k, "s": round(t, 2)}), flush=True)
+    gen_s = time.perf_counter() - t0
+    sur = S.Surrogate(hidden=512, layers=4, max_batch=64, seed=args.seed)
+    t1 = time.perf_counter()
+    hist = sur.fit(np.array(feats), np.array(targets), epochs=args.epochs, batch=min(32, len(train)), lr=1e-3)
+    train_s = time.perf_counter() - t1
+    print(json.dumps({"train_loss_first": hist[0], "train_loss_last": hist[-1], "train_s": round(train_s, 1)}),
+          flush=True)
+    rows = []
+    for i, d in enumerate(held):
+        p = S.problem(d, args.n)
+        _, kp, enp, tp = S.solve(p, tol=args.tol, max_steps=args.max_steps)
+        t2 = time.perf_counter()
+        g = sur.guess(d, args.n)
+        ti = time.perf_counter() - t2
+        _, ks, ens, ts = S.solve(p, g, tol=args.tol, max_steps=args.max_steps)
+        row = {"draw": i, "plain_steps": kp, "plain_s": round(tp, 3), "surrogate_steps": ks,
+               "surrogate_s": round(ts + ti, 3), "inference_s": round(ti, 4), "energy_rel_gap": abs(ens / enp - 1),
+               "plain_capped": kp >= args.max_steps, "surrogate_capped": ks >= args.max_steps}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    sur.close()
+    sp = np.array([r["plain_steps"] for r in rows], float)
+    ss = np.array([r["surrogate_steps"] for r in rows], float)
+    tp = np.array([r["plain_s"] for r in rows])
+    ts = np.array([r["surrogate_s"] for r in rows])
+    out = {
+        "metric": "ground-state time-to-TOL, surrogate guess vs plain Gaussian guess",
+        "value": round(float(tp.sum() / ts.sum()), 3), "unit": "x (total wall-clock ratio)",
+        "higher_is_better": True,
+        "median_step_ratio": round(float(np.median(sp / ss)), 3),
+        "total_step_ratio": round(float(sp.sum() / ss.sum()), 3),
+        "max_energy_rel_gap": float(max(r["energy_rel_gap"] for r in rows)),
+        "config": {"workload": "q2D spin-1 imaginary time, %d^2 over a 25.6 box, DT=0.001, TOL=%g, STP=100; "
+                               "draws: GAMMAX,GAMMAY~U[0,2], A0~U[99.8,103.8], A2-A0~U[-5,5] a_B, "
+                               "NATOMS~logU[1e3,1e5], aspect~U[0.8,1.25], trap 20 Hz in plane, 400 Hz out"
+                               % (args.n, args.tol),
+                   "network": "6-512x4-24576 (3 complex components on 64^2), BF16 MFMA, FP32 accumulation",
+                   "train_states": len(train), "train_tol": args.train_tol, "held_out": len(held),
+                   "max_steps": args.max_steps},
+        "data_generation_s": round(gen_s, 1), "training_s": round(train_s, 1),
+        "dtype": "fp64 solves, bf16/fp32 network",
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +182,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy", default=None, metavar="STEPS,...",
                     help="instead of timing, compare FP32 with FP64 propagation at these step counts")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="instead of the propagator throughput, the learned initial guess's time-to-TOL gain")
+    ap.add_argument("--train", type=int, default=160)
+    ap.add_argument("--held-out", type=int, default=16)
+    ap.add_argument("--train-tol", type=float, default=1e-6)
+    ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--max-steps", type=int, default=400000)
+    ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -129,6 +198,8 @@ def main():
 
     if args.accuracy:
         return accuracy(args)
+    if args.surrogate:
+        return surrogate_bench(args)
 
     from fortress_amd import fgpe
     from fortress_amd.abi import FGPE_FP32

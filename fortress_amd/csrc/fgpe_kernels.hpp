@@ -234,20 +234,27 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
       else
         se2_point<C, T>(pt, ph, dt, ph.dim == 1 ? (T)0.5 : (T)1, sizeof(T) == 4 ? (T)1e-8 : (T)1e-17);
       sp_point<SPIN, C, T>(pt, tb.vx[x] + vyz, ph, dt);
-      const double wx = tb.wx[x];
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
         v[m][q] = pt[m];
-        acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
       });
+      if (flags & XF_NORM) {
+        const double wx = tb.wx[x];
+        SFor<0, MS>::run([&](auto mi) {
+          constexpr int m = decltype(mi)::value;
+          acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
+        });
+      }
       // one point at a time: keeps the SE temporaries of a single point live
       __builtin_amdgcn_sched_barrier(0);
     });
-    SFor<0, MS>::run([&](auto mi) {
-      constexpr int m = decltype(mi)::value;
-      acc[m] = valid ? acc[m] * wyz : 0.0;
-    });
-    if (flags & XF_NORM) block_sum_store<MS>(acc, red, partials);
+    if (flags & XF_NORM) {
+      SFor<0, MS>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        acc[m] = valid ? acc[m] * wyz : 0.0;
+      });
+      block_sum_store<MS>(acc, red, partials);
+    }
   }
   if (flags & XF_FWD) {
     SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
