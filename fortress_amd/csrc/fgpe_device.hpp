@@ -502,26 +502,37 @@ __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T d
       // spin part delta_m = DT es_m:
       //   exp(-i theta_m) - 1 = db + dm + db dm,
       // db from one sincos of theta_b/2 (-2 sin^2 - i sin, plus the lo term),
-      // dm from short series when |delta_m| <= 1/4 (else one more sincos).
+      // dm from short series when every |delta_m| <= 1/4 (else one sincos each).
       T s2, c2;
       sincos_t(bh * (T)0.5, &s2, &c2);
       const T sn = (T)2 * s2 * c2, cm = (T)-2 * s2 * s2;
       const C db = mk<C>(cm - bl * sn, -(sn + bl * ((T)1 + cm)));
+      T dl[MS], big = 0;
 #pragma unroll
       for (int m = 0; m < MS; ++m) {
-        const T dl = dt * es[m];
-        C dm;
-        if (fabs(dl) <= (T)0.25) {
-          const T q = dl * dl;
-          const T sd = dl * ((T)1 - q * ((T)1 / 6) * ((T)1 - q * ((T)1 / 20) * ((T)1 - q * ((T)1 / 42))));
+        dl[m] = dt * es[m];
+        big = fmax(big, fabs(dl[m]));
+      }
+      C dm[MS];
+      if (big <= (T)0.25) {
+#pragma unroll
+        for (int m = 0; m < MS; ++m) {
+          const T q = dl[m] * dl[m];
+          const T sd = dl[m] * ((T)1 - q * ((T)1 / 6) * ((T)1 - q * ((T)1 / 20) * ((T)1 - q * ((T)1 / 42))));
           const T cd = -q * (T)0.5 * ((T)1 - q * ((T)1 / 12) * ((T)1 - q * ((T)1 / 30) * ((T)1 - q * ((T)1 / 56))));
-          dm = mk<C>(cd, -sd);
-        } else {
-          T s3, c3;
-          sincos_t(dl * (T)0.5, &s3, &c3);
-          dm = mk<C>((T)-2 * s3 * s3, (T)-2 * s3 * c3);
+          dm[m] = mk<C>(cd, -sd);
         }
-        const C d = cadd(cadd(db, dm), cmul(db, dm));
+      } else {
+#pragma unroll
+        for (int m = 0; m < MS; ++m) {
+          T s3, c3;
+          sincos_t(dl[m] * (T)0.5, &s3, &c3);
+          dm[m] = mk<C>((T)-2 * s3 * s3, (T)-2 * s3 * c3);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const C d = cadd(cadd(db, dm[m]), cmul(db, dm[m]));
         ph[m] = cadd(ph[m], cmul(ph[m], d));
       }
     }
@@ -556,13 +567,45 @@ template <typename C> __host__ __device__ __forceinline__ void hmul2(const C* c,
   o[4] = cadd(cmulc(v[2], c[4]), cmulc(v[3], c[5]));
 }
 
+// acc + a b and acc + a conj(b)
+template <typename C> __host__ __device__ __forceinline__ C cmac(C acc, C a, C b) {
+  return mk<C>(fma(a.x, b.x, fma(-a.y, b.y, acc.x)), fma(a.x, b.y, fma(a.y, b.x, acc.y)));
+}
+template <typename C> __host__ __device__ __forceinline__ C cmacc(C acc, C a, C b) {
+  return mk<C>(fma(a.x, b.x, fma(a.y, b.y, acc.x)), fma(a.y, b.x, fma(-a.x, b.y, acc.y)));
+}
+
+// hmul2 with every product accumulated by FMA (12 complex products, 48 FMAs)
+template <typename C> __host__ __device__ __forceinline__ void hmul2f(const C* c, const C* v, C* o) {
+  o[0] = cmac(cmul(c[0], v[1]), c[1], v[2]);
+  o[1] = cmac(cmulc(v[0], c[0]), c[2], v[2]);
+  o[2] = cmac(cmac(cmacc(cmulc(v[0], c[1]), v[1], c[2]), c[3], v[3]), c[4], v[4]);
+  o[3] = cmac(cmulc(v[2], c[3]), c[5], v[4]);
+  o[4] = cmacc(cmulc(v[2], c[4]), v[3], c[5]);
+}
+
+// 1/k for a small positive integer k (one correctly rounded or 1-ulp value;
+// the terms it scales are far below the rounding of the state)
+__host__ __device__ __forceinline__ float recip_k(int k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf((float)k);
+#else
+  return 1.0f / (float)k;
+#endif
+}
+
 // SE spin-2 (SRC/cgpe3d.f90:1673-1741): predictor with factor cpred (1/2 in
 // 1D, 1 in 2D/3D), averaged couplings, then exp(-i CDT Hbar) phi.  ZHEEV's
 // V diag(exp(-i CDT w)) V^H is replaced by its Taylor series applied to the
 // vector, summed to rounding (the matrix exponential is unique) apart from the
 // zeroth term, which is added last so rounding scales with the change; when
 // |CDT|*||Hbar||_inf exceeds 1/2 the exponential is split into equal
-// sub-steps.
+// sub-steps.  FP64 stops the series when a term falls below tol relative to
+// the vector.  FP32 fixes the number of terms K beforehand from the bound
+// bs = |CDT| h ||Hbar||_inf (h the sub-step): the remainder after K terms is
+// at most bs^(K+1)/(K+1)! e^bs ||phi||, and K is the smallest with
+// bs^(K+1)/(K+1)! <= 2^-34, far below FP32 rounding, and a term costs one
+// sparse matrix-vector product with fused multiply-adds and one scaling.
 template <typename C, typename T>
 __host__ __device__ __forceinline__ void se2_point(C* ph, const Phys& p, T dt, T cpred, T tol) {
   C c0[6], c1[6], hv[5], p1[5];
@@ -582,6 +625,36 @@ __host__ __device__ __forceinline__ void se2_point(C* ph, const Phys& p, T dt, T
   // bounded sub-step count (a non-finite bound must not spin the loop)
   int nsub = b > (T)0.5 ? (b < (T)32 ? (int)ceil(b / (T)0.5) : 64) : 1;
   T h = dt / (T)nsub;
+  if constexpr (sizeof(T) == 4) {
+    const T bs = b / (T)nsub;
+    int K = 1;
+    T rem = bs * bs * (T)0.5;
+    while (rem > (T)5.8207660913467407e-11 && K < 30) {
+      ++K;
+      rem *= bs * recip_k(K + 1);
+    }
+    for (int s = 0; s < nsub; ++s) {
+      C term[5], sum[5];
+      hmul2f(c0, ph, hv);
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        term[m] = mul_micdt<C, T>(hv[m], h, p.imag);
+        sum[m] = term[m];
+      }
+      for (int k = 2; k <= K; ++k) {
+        hmul2f(c0, term, hv);
+        const T f = h * recip_k(k);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          term[m] = mul_micdt<C, T>(hv[m], f, p.imag);
+          sum[m] = cadd(sum[m], term[m]);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 5; ++m) ph[m] = cadd(ph[m], sum[m]);
+    }
+    return;
+  }
   for (int s = 0; s < nsub; ++s) {
     C term[5], sum[5];
     T smax = 0;

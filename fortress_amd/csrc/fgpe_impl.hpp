@@ -40,6 +40,9 @@ template <> struct Cfg<float> {
   static constexpr int QK = 8;   // pass C (k-space lines, all components)
   static constexpr int Q1 = 16;  // q1D whole-step kernel
   static constexpr int TX = 16;  // x columns per strided-line block (128 B rows)
+  // x columns per pass-C block: 256 threads, so three blocks share a CU and
+  // their load, transform and store phases overlap
+  static constexpr int TXK = 8;
 };
 template <> struct Cfg<double> {
   static constexpr int QX = 8;
@@ -47,6 +50,7 @@ template <> struct Cfg<double> {
   static constexpr int QK = 4;
   static constexpr int Q1 = 8;
   static constexpr int TX = 8;
+  static constexpr int TXK = 8;
 };
 
 // ---------------------------------------------------- host-side tables

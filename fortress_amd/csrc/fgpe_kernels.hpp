@@ -186,7 +186,8 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
-  double* red = reinterpret_cast<double*>(lds_tw + TWS);
+  R_t<T>* lds_kex = lds_tw + TWS;
+  double* red = reinterpret_cast<double*>(lds_kex + N);
   C* lds = reinterpret_cast<C*>(red + 16 * MS);
 
   const int lpw = blockDim.x / P;
@@ -198,6 +199,8 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
   C* my = lds + tl * LS;
 
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
+  if (flags & XF_FWD)
+    for (int i = threadIdx.x; i < N; i += blockDim.x) lds_kex[i] = tb.kex[i];
 
   C v[MS][Q];
   const unsigned int voff = (unsigned int)((line * N + j) * (long)sizeof(C));
@@ -218,9 +221,6 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
   if (flags & XF_PHYS) {
     const T dt = (T)ph.dt;
     const T vyz = tb.vy[y] + tb.vz[z];
-    const double wyz = tb.wy[y] * tb.wz[z];
-    double acc[MS];
-    SFor<0, MS>::run([&](auto mi) { acc[decltype(mi)::value] = 0.0; });
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;
       const int x = j + P * q;
@@ -238,17 +238,23 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
         constexpr int m = decltype(mi)::value;
         v[m][q] = pt[m];
       });
-      if (flags & XF_NORM) {
-        const double wx = tb.wx[x];
-        SFor<0, MS>::run([&](auto mi) {
-          constexpr int m = decltype(mi)::value;
-          acc[m] += wx * ((double)pt[m].x * (double)pt[m].x + (double)pt[m].y * (double)pt[m].y);
-        });
-      }
       // one point at a time: keeps the SE temporaries of a single point live
       __builtin_amdgcn_sched_barrier(0);
     });
     if (flags & XF_NORM) {
+      // NORMC partials after the point loop, so the FP64 sums are not live
+      // across SE and SP
+      const double wyz = tb.wy[y] * tb.wz[z];
+      double acc[MS];
+      SFor<0, MS>::run([&](auto mi) { acc[decltype(mi)::value] = 0.0; });
+      SFor<0, Q>::run([&](auto qi) {
+        constexpr int q = decltype(qi)::value;
+        const double wx = tb.wx[j + P * q];
+        SFor<0, MS>::run([&](auto mi) {
+          constexpr int m = decltype(mi)::value;
+          acc[m] += wx * ((double)v[m][q].x * (double)v[m][q].x + (double)v[m][q].y * (double)v[m][q].y);
+        });
+      });
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
         acc[m] = valid ? acc[m] * wyz : 0.0;
@@ -256,26 +262,29 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
       block_sum_store<MS>(acc, red, partials);
     }
   }
-  if (flags & XF_FWD) {
-    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
-    // the kx factor of the next step's KE
-    SFor<0, Q>::run([&](auto qi) {
-      constexpr int q = decltype(qi)::value;
-      const R_t<T> f = tb.kex[j + P * q];
-      SFor<0, MS>::run([&](auto mi) {
-        constexpr int m = decltype(mi)::value;
-        v[m][q] = rot1(v[m][q], f);
-      });
-    });
-  }
-  SFor<0, MS>::run([&](auto mi) {
+  auto store = [&](auto mi) {
     constexpr int m = decltype(mi)::value;
     const CompBuf<C> buf(phi + m * g.npts, g.npts);
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;
       buf.st(v[m][q], voff, (unsigned int)(P * q * sizeof(C)));
     });
-  });
+  };
+  if (flags & XF_FWD) {
+    // forward FFT, the kx factor of the next step's KE, store: one component
+    // at a time, so the stores overlap the remaining transforms
+    SFor<0, MS>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      line_fft<N, Q, -1>(v[m], my, lds_tw, j);
+      SFor<0, Q>::run([&](auto qi) {
+        constexpr int q = decltype(qi)::value;
+        v[m][q] = rot1(v[m][q], lds_kex[j + P * q]);
+      });
+      store(mi);
+    });
+  } else {
+    SFor<0, MS>::run(store);
+  }
 }
 
 // ------------------------------------------------------- pass B/D (y)
@@ -346,6 +355,12 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
   for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
   const unsigned int voff = (unsigned int)(((long)o * g.nx + x + (long)j * lstride) * (long)sizeof(C));
   const unsigned int sstep = (unsigned int)((long)P * lstride * (long)sizeof(C));
+  const T dt = (T)ph.dt;
+  // in 3D the SOC factors depend on (kx, ky) only: formed before the line is
+  // loaded, so their transcendental code does not run while it is live
+  C e1, dm1[MS];
+  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, dm1);
+  __builtin_amdgcn_sched_barrier(0);
   C v[MS][Q];
   SFor<0, MS>::run([&](auto mi) {
     constexpr int m = decltype(mi)::value;
@@ -360,9 +375,6 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
   __syncthreads();
   C* my = lds + l * LS;
   SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
-  const T dt = (T)ph.dt;
-  C e1, dm1[MS];
-  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, dm1);
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;
     const int pos = j + P * q;
@@ -381,9 +393,11 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
     });
     __builtin_amdgcn_sched_barrier(0);
   });
-  SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j); });
+  // each component is stored as soon as its inverse FFT is done, so the
+  // stores overlap the remaining transforms
   SFor<0, MS>::run([&](auto mi) {
     constexpr int m = decltype(mi)::value;
+    line_fft<N, Q, +1>(v[m], my, lds_tw, j);
     const CompBuf<C> buf(phi + m * g.npts, g.npts);
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;

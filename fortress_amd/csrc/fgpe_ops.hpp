@@ -20,7 +20,7 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
   const int lpw = threads / P;
   const long nlines = (long)h->ny * h->nz;
   const int blocks = (int)((nlines + lpw - 1) / lpw);
-  const size_t lds = TWS * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
+  const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
   if (flags & XF_PHYS) {
     int rc = ensure_partials(h, (size_t)blocks * MS);
@@ -59,7 +59,7 @@ template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
 }
 
 template <typename T, int SPIN, int DIM, int N> int launch_k(fgpe_handle* h, int scale) {
-  constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TX;
+  constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TXK;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(R_t<T>) + (size_t)MS * N * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);

@@ -3,12 +3,13 @@
 A small network (the production architecture with narrower hidden layers) is
 trained on ground states the solver produces for seeded parameter draws.  On
 held-out draws its guess (after NORMT) must lie closer to the converged
-ground state than the Gaussian guess does, measured as the relative L2
-distance of the spinors; the imaginary-time solve from either guess is run to
-TOL as well and its step counts and energies are printed.  (Step counts are
-reported, not asserted: with a small training set some held-out guesses fall
-into another branch of the spin texture; the benchmark, bench.py --surrogate,
-reports the time-to-TOL ratio on the production grid.)
+ground state than the Gaussian guess does, and the imaginary-time solve from
+it must reach TOL in fewer steps in total.  Closeness is measured by
+quantities that do not depend on the global phase of the spinor (the ground
+state is fixed only up to it): the relative L2 distance of the component
+densities |phi_m|^2, and the infidelity 1 - |<guess, phi_gs>| of the
+normalised spinors, which is printed.  The benchmark (bench.py --surrogate)
+reports the time-to-TOL ratio on the production grid.
 """
 import numpy as np
 import pytest
@@ -29,7 +30,7 @@ def normalised(params, phi):
         return s.download()
 
 
-def test_surrogate_guess_is_closer_to_the_ground_state():
+def test_surrogate_guess_is_closer_and_converges_sooner():
     rng = np.random.default_rng(77)
     train = [S.draw(rng) for _ in range(24)]
     held = [S.draw(rng) for _ in range(3)]
@@ -42,17 +43,22 @@ def test_surrogate_guess_is_closer_to_the_ground_state():
     hist = sur.fit(np.array(feats), np.array(targets), epochs=200, batch=8, lr=1e-3)
     print("train loss", hist[0], "->", hist[-1], flush=True)
     assert hist[-1] < 0.2 * hist[0]
-    d_plain, d_sur = [], []
+    d_plain, d_sur, steps_plain, steps_sur = [], [], [], []
     for d in held:
         p = S.problem(d, S.COARSE)
         gs, kp, enp, _ = S.solve(p, tol=1e-7, max_steps=200000)
         g0 = normalised(p, None)
         g1 = normalised(p, sur.guess(d, S.COARSE))
-        dist = lambda g: float(np.linalg.norm(g - gs) / np.linalg.norm(gs))
+        dist = lambda g: float(np.linalg.norm(np.abs(g) ** 2 - np.abs(gs) ** 2) / np.linalg.norm(np.abs(gs) ** 2))
+        infid = lambda g: 1.0 - abs(np.vdot(g, gs)) / (np.linalg.norm(g) * np.linalg.norm(gs))
         d_plain.append(dist(g0))
         d_sur.append(dist(g1))
         _, ks, ens, _ = S.solve(p, g1, tol=1e-7, max_steps=200000)
-        print("held-out draw: distance to ground state plain %.3f surrogate %.3f; steps to TOL plain %d "
-              "surrogate %d; energy gap %.1e" % (d_plain[-1], d_sur[-1], kp, ks, abs(ens / enp - 1)), flush=True)
+        steps_plain.append(kp)
+        steps_sur.append(ks)
+        print("held-out draw: density distance plain %.3f surrogate %.3f; infidelity plain %.3f surrogate %.3f; "
+              "steps to TOL plain %d surrogate %d; energy gap %.1e"
+              % (d_plain[-1], d_sur[-1], infid(g0), infid(g1), kp, ks, abs(ens / enp - 1)), flush=True)
     sur.close()
     assert np.mean(d_sur) < np.mean(d_plain)
+    assert sum(steps_sur) < sum(steps_plain)
