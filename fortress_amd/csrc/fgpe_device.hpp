@@ -502,7 +502,9 @@ __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T d
       // spin part delta_m = DT es_m:
       //   exp(-i theta_m) - 1 = db + dm + db dm,
       // db from one sincos of theta_b/2 (-2 sin^2 - i sin, plus the lo term),
-      // dm from short series when every |delta_m| <= 1/4 (else one sincos each).
+      // dm from series when every |delta_m| <= 1/4 (two terms when <= 2^-10;
+      // the truncation is below 1e-13 relative either way), else one sincos
+      // each.
       T s2, c2;
       sincos_t(bh * (T)0.5, &s2, &c2);
       const T sn = (T)2 * s2 * c2, cm = (T)-2 * s2 * s2;
@@ -514,7 +516,13 @@ __host__ __device__ __forceinline__ void sp_point(C* ph, T v, const Phys& p, T d
         big = fmax(big, fabs(dl[m]));
       }
       C dm[MS];
-      if (big <= (T)0.25) {
+      if (big <= (T)0.0009765625) {
+#pragma unroll
+        for (int m = 0; m < MS; ++m) {
+          const T q = dl[m] * dl[m];
+          dm[m] = mk<C>(-q * (T)0.5 * ((T)1 - q * ((T)1 / 12)), -dl[m] * ((T)1 - q * ((T)1 / 6)));
+        }
+      } else if (big <= (T)0.25) {
 #pragma unroll
         for (int m = 0; m < MS; ++m) {
           const T q = dl[m] * dl[m];
@@ -608,42 +616,62 @@ __host__ __device__ __forceinline__ float recip_k(int k) {
 // sparse matrix-vector product with fused multiply-adds and one scaling.
 template <typename C, typename T>
 __host__ __device__ __forceinline__ void se2_point(C* ph, const Phys& p, T dt, T cpred, T tol) {
+  constexpr bool F32 = sizeof(T) == 4;
   C c0[6], c1[6], hv[5], p1[5];
   mat_c2<C, T>(ph, p, c0);
-  hmul2(c0, ph, hv);
+  if constexpr (F32)
+    hmul2f(c0, ph, hv);
+  else
+    hmul2(c0, ph, hv);
 #pragma unroll
   for (int m = 0; m < 5; ++m) p1[m] = cadd(ph[m], mul_micdt<C, T>(hv[m], dt * cpred, p.imag));
   mat_c2<C, T>(p1, p, c1);
+  // averaged couplings; FP32 keeps the sum and folds the 1/2 into the time
+  // factor (an exact power of two either way)
+  const T half = F32 ? (T)0.5 : (T)1;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) c0[i] = cscale(cadd(c0[i], c1[i]), (T)0.5);
+  for (int i = 0; i < 6; ++i) c0[i] = F32 ? cadd(c0[i], c1[i]) : cscale(cadd(c0[i], c1[i]), (T)0.5);
   // infinity-norm bound with |re|+|im|
   T n[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) n[i] = fabs(c0[i].x) + fabs(c0[i].y);
   T rows = fmax(fmax(n[0] + n[1], n[0] + n[2]), fmax(n[1] + n[2] + n[3] + n[4], fmax(n[3] + n[5], n[4] + n[5])));
-  T b = dt * rows;
+  T b = dt * half * rows;
   // bounded sub-step count (a non-finite bound must not spin the loop)
   int nsub = b > (T)0.5 ? (b < (T)32 ? (int)ceil(b / (T)0.5) : 64) : 1;
-  T h = dt / (T)nsub;
+  T h = dt, bs = b;  // per sub-step
+  if (nsub > 1) {
+    h = dt / (T)nsub;
+    bs = b / (T)nsub;
+  }
   if constexpr (sizeof(T) == 4) {
-    const T bs = b / (T)nsub;
+    // K terms suffice when bs <= t_K = ((K+1)! 2^-34)^(1/(K+1)) (rounded
+    // down); past t_9 the bound is followed term by term
+    constexpr float tk[9] = {1.077880e-05f, 7.035192e-04f, 6.107498e-03f, 2.335596e-02f, 5.887771e-02f,
+                             1.165034e-01f, 1.976415e-01f, 3.020595e-01f, 4.285878e-01f};
     int K = 1;
-    T rem = bs * bs * (T)0.5;
-    while (rem > (T)5.8207660913467407e-11 && K < 30) {
-      ++K;
-      rem *= bs * recip_k(K + 1);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) K += bs > tk[i] ? 1 : 0;
+    if (bs > tk[8]) {
+      K = 1;
+      T rem = bs * bs * (T)0.5;
+      while (rem > (T)5.8207660913467407e-11 && K < 30) {
+        ++K;
+        rem *= bs * recip_k(K + 1);
+      }
     }
+    const T hh = h * half;
     for (int s = 0; s < nsub; ++s) {
       C term[5], sum[5];
       hmul2f(c0, ph, hv);
 #pragma unroll
       for (int m = 0; m < 5; ++m) {
-        term[m] = mul_micdt<C, T>(hv[m], h, p.imag);
+        term[m] = mul_micdt<C, T>(hv[m], hh, p.imag);
         sum[m] = term[m];
       }
       for (int k = 2; k <= K; ++k) {
         hmul2f(c0, term, hv);
-        const T f = h * recip_k(k);
+        const T f = hh * recip_k(k);
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
           term[m] = mul_micdt<C, T>(hv[m], f, p.imag);

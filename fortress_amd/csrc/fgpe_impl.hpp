@@ -35,7 +35,9 @@ constexpr double EPS = 1.0e-40;
 // passes hold 2*SPIN+1 complex values per point in registers.
 template <typename T> struct Cfg;
 template <> struct Cfg<float> {
-  static constexpr int QX = 8;  // pass A (x lines, all components)
+  // pass A (x lines, all components): 4 points per thread keep it at <= 128
+  // VGPRs (4 waves per SIMD); its SE/SP work is latency-bound, not VALU-bound
+  static constexpr int QX = 4;
   static constexpr int QY = 16;  // passes B/D (y lines, one component)
   static constexpr int QK = 8;   // pass C (k-space lines, all components)
   static constexpr int Q1 = 16;  // q1D whole-step kernel
