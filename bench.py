@@ -171,6 +171,70 @@ def surrogate_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_bench(args):
+    """Batched q2D spin-1 sweep: args.instances imaginary-time instances with
+    seeded grid sizes from {64,128,192,256}^2, DX = 0.1, DT = 0.001, 87Rb spin
+    1 with A0 = 101.8 a_B, A2 - A0 ~ U[-5,5], GAMMAX, GAMMAY ~ U[0,2], NATOMS
+    ~ logU[1e3,1e5], trap 20 Hz in-plane with aspect ~ U[0.8,1.25], FP32.
+    After the autotuner, args.steps batched steps are timed; the same steps
+    of a sample of the instances one at a time (fgpe_run on a single-instance
+    handle each) give the unbatched rate beside it."""
+    import numpy as np
+    from fortress_amd import fgpe
+    from fortress_amd.abi import FGPE_FP32, make_input
+    rng = np.random.default_rng(args.seed)
+    inps = []
+    for _ in range(args.instances):
+        n = int(rng.choice([64, 128, 192, 256]))
+        a0 = 101.8
+        inps.append(make_input(2, 1, (n, n), (0.1, 0.1), 1e-3, 87.0, (a0, a0 + rng.uniform(-5, 5), 0.0),
+                               int(round(10 ** rng.uniform(3, 5))), (20.0, 20.0 * rng.uniform(0.8, 1.25), 400.0),
+                               (rng.uniform(0, 2), rng.uniform(0, 2)), switch_im=1))
+    params = [fgpe.derive(i, FGPE_FP32)[0] for i in inps]
+    sizes = [p.n[0] for p in params]
+    pts = sum(n * n for n in sizes)
+    with fgpe.Batch(params) as b:
+        b.initial_guess()
+        ms_tune, chosen = b.tune(3)
+        b.run(args.warmup)
+        b.run(args.steps)
+        ms = b.last_ms()
+    rate = args.instances * args.steps / (ms * 1e-3)
+    # one at a time: the power-of-two instances (the single-instance path has
+    # no 192-point lines), a bounded sample
+    sample = [i for i, n in enumerate(sizes) if n != 192][: args.sample]
+    t_single = 0.0
+    for i in sample:
+        with fgpe.Solver(params[i]) as s:
+            s.initial_guess()
+            s.normalize()
+            s.run(args.warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(args.steps)
+            s.sync()
+            t_single += time.perf_counter() - t0
+    single_rate = len(sample) * args.steps / t_single
+    # HBM bytes per step: two passes, each reading and writing the 3-component FP32 state
+    bytes_step = 2 * 2 * pts * 3 * 8
+    line = {
+        "metric": "batched q2D spin-1 sweep: instance-steps/s",
+        "value": round(rate, 1), "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 propagation, fp64 reductions",
+        "data": "synthetic seeded parameter sweep, Gaussian initial guesses",
+        "config": {"workload": "%d q2D spin-1 imaginary-time instances, grids from {64,128,192,256}^2, DX=0.1, "
+                               "DT=0.001, autotuned schedules" % args.instances,
+                   "instances_per_size": {str(n): sizes.count(n) for n in (64, 128, 192, 256)},
+                   "tune_ms": [round(x, 4) for x in ms_tune], "chosen": list(chosen)},
+        "step_GBps": round(bytes_step * args.steps / (ms * 1e-3) / 1e9, 1),
+        "unbatched": {"value": round(single_rate, 1), "unit": "instance-steps/s",
+                      "sample": "%d power-of-two instances, one handle and fgpe_run each" % len(sample)},
+        "speedup_vs_unbatched": round(rate / single_rate, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +255,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--max-steps", type=int, default=400000)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--batch", action="store_true",
+                    help="instead of the 3D propagator, the batched q2D parameter sweep")
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--sample", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -198,6 +266,8 @@ def main():
 
     if args.accuracy:
         return accuracy(args)
+    if args.batch:
+        return batch_bench(args)
     if args.surrogate:
         return surrogate_bench(args)
 

@@ -122,8 +122,9 @@ template <int M> __device__ __forceinline__ void wave_sum(double (&a)[M]) {
       for (int m = 0; m < M; ++m) a[m] += __shfl_xor(a[m], off, 64);
 }
 
-// sum a[M] over the block and write out[blockIdx.x*M + m]; red: >= 16*M doubles
-template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M], double* red, double* out) {
+// sum a[M] over the block and write out[blk*M + m] (blk: the block index, by
+// default blockIdx.x); red: >= 16*M doubles
+template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M], double* red, double* out, long blk) {
   wave_sum<M>(a);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   __syncthreads();
@@ -134,16 +135,19 @@ template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M],
   if (threadIdx.x < M) {
     double s = 0;
     for (int w = 0; w < nw; ++w) s += red[w * M + threadIdx.x];
-    out[(long)blockIdx.x * M + threadIdx.x] = s;
+    out[blk * M + threadIdx.x] = s;
   }
+}
+template <int M> __device__ __forceinline__ void block_sum_store(double (&a)[M], double* red, double* out) {
+  block_sum_store<M>(a, red, out, (long)blockIdx.x);
 }
 
 // Sum the NORMC partials of a pass in a fixed order (deterministic), store
 // the norms and the per-component renormalisation scale of the step.
 template <typename T, int MS>
-__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, int nparts, FinArgs f,
-                                                   double* norm_out, C_t<T>* cscale, double* cscale64, double* sig,
-                                                   int* err) {
+__device__ __forceinline__ void finalize_block(const double* __restrict__ partials, int nparts, const FinArgs& f,
+                                               double* norm_out, C_t<T>* cscale, double* cscale64, double* sig,
+                                               int* err) {
   __shared__ double red[16 * MS];
   double a[MS];
   SFor<0, MS>::run([&](auto mi) { a[decltype(mi)::value] = 0.0; });
@@ -170,6 +174,12 @@ __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ pa
     }
   }
 }
+template <typename T, int MS>
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, int nparts, FinArgs f,
+                                                   double* norm_out, C_t<T>* cscale, double* cscale64, double* sig,
+                                                   int* err) {
+  finalize_block<T, MS>(partials, nparts, f, norm_out, cscale, cscale64, sig, err);
+}
 
 // ------------------------------------------------------------ pass A (x)
 // Lines along x, all components.  flags: XF_INV inverse FFT first, XF_PHYS
@@ -177,8 +187,8 @@ __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ pa
 // XF_SCALE multiply by the pending scale on load, XF_NORM (with XF_PHYS) write
 // the block's NORMC partials.
 template <typename T, int SPIN, int N, int Q>
-__global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
-                                               double* __restrict__ partials) {
+__device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
+                                            const Phys& ph, int flags, double* __restrict__ partials) {
   using C = C_t<T>;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int P = N / Q;
@@ -192,7 +202,7 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
 
   const int lpw = blockDim.x / P;
   const int tl = threadIdx.x / P, j = threadIdx.x % P;
-  const long line = (long)blockIdx.x * lpw + tl;
+  const long line = (long)blk * lpw + tl;
   const long nlines = (long)g.ny * g.nz;
   const bool valid = line < nlines;
   const int y = valid ? (int)(line % g.ny) : 0, z = valid ? (int)(line / g.ny) : 0;
@@ -259,7 +269,7 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
         constexpr int m = decltype(mi)::value;
         acc[m] = valid ? acc[m] * wyz : 0.0;
       });
-      block_sum_store<MS>(acc, red, partials);
+      block_sum_store<MS>(acc, red, partials, (long)blk);
     }
   }
   auto store = [&](auto mi) {
@@ -285,6 +295,11 @@ __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g,
   } else {
     SFor<0, MS>::run(store);
   }
+}
+template <typename T, int SPIN, int N, int Q>
+__global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
+                                               double* __restrict__ partials) {
+  xpass_block<T, SPIN, N, Q>(blockIdx.x, phi, g, tb, ph, flags, partials);
 }
 
 // ------------------------------------------------------- pass B/D (y)
@@ -336,7 +351,8 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi,
 // KE (SRC/cgpe3d.f90:875-899), SOC (904-1004), inverse FFT.  A block owns TX
 // consecutive x (and one y in 3D).
 template <typename T, int SPIN, int DIM, int N, int Q, int TX>
-__global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
+__device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
+                                            const Phys& ph, int scale) {
   using C = C_t<T>;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int P = N / Q;
@@ -347,8 +363,8 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
   R_t<T>* lds_ke = lds_tw + TWS;
   C* lds = reinterpret_cast<C*>(lds_ke + MS * N);
   const int l = threadIdx.x % TX, j = threadIdx.x / TX;
-  const int x = blockIdx.x * TX + l;
-  const int o = DIM == 3 ? blockIdx.y : 0;  // y index in 3D
+  const int x = bx * TX + l;
+  const int o = DIM == 3 ? by : 0;  // y index in 3D
   const long lstride = DIM == 3 ? (long)g.nx * g.ny : (long)g.nx;
   const R_t<T>* tw = DIM == 3 ? tb.twz : tb.twy;
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tw[i];
@@ -405,7 +421,10 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi,
     });
   });
 }
-
+template <typename T, int SPIN, int DIM, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
+  kpass_block<T, SPIN, DIM, N, Q, TX>(blockIdx.x, blockIdx.y, phi, g, tb, ph, scale);
+}
 
 // ------------------------------------------------------------------- 1D
 // One workgroup advances the whole q1D problem nsteps times: forward FFT,

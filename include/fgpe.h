@@ -177,14 +177,12 @@ int fgpe_last_error(char* buf, size_t len);
  * set); each instance follows exactly the single-instance step of
  * fgpe_run (SRC/cgpe2d.f90:440-485) and the convergence test of cgpe.x
  * (CONV = max|PHI - PHI_OLD|/(STP*2*DT), CONV_EN = |EN - EN_OLD|/(STP*2*DT),
- * converged when either is <= TOL; SRC/cgpe2d.f90:488-495).  One persistent
- * launch advances every unconverged instance by STP steps: blocks take
- * instances from a work queue, largest grids first, and run all of an
- * instance's passes themselves.  Instances: dim 2, spin 1, square grids of
- * 64, 128, 192 or 256 points per axis, one precision for the whole batch.
- *
- * Disabled in this build: fgpe_batch_create returns FGPE_EINVAL (the batch
- * kernel faulted on its first device run; the cause is not yet found).
+ * converged when either is <= TOL; SRC/cgpe2d.f90:488-495).  Instances are
+ * grouped by grid size and every pass of a step is one launch per size over
+ * all of that size's unconverged instances (blocks map to instance x lines),
+ * so a step of any number of instances costs a fixed handful of launches.
+ * Instances: dim 2, spin 1, square grids of 64, 128, 192 or 256 points per
+ * axis, one precision and one time mode (real or imaginary) for the batch.
  */
 typedef struct fgpe_batch fgpe_batch;
 
@@ -202,8 +200,11 @@ typedef struct fgpe_batch_result {
   double norm[5];   /* NORMC of the last step, before renormalisation */
 } fgpe_batch_result;
 
-/* Points-per-thread schedules of the batch kernel, per line length. */
-#define FGPE_BATCH_NSCHED 8   /* 64:Q8,Q4  128:Q8,Q16  192:Q12,Q6  256:Q8,Q16 */
+/* Schedules (points per thread of the x pass / the k pass), two per grid
+ * size: s = 2*c + v for c = 0..3 (64, 128, 192, 256).  FP32: 64 (4/8, 8/4),
+ * 128 (4/8, 8/16), 192 (12/12, 6/6), 256 (4/8, 8/16); FP64: 64 (8/4, 4/8),
+ * 128 (8/4, 4/8), 192 (6/6, 12/6), 256 (8/4, 4/8). */
+#define FGPE_BATCH_NSCHED 8
 
 /* Allocate a batch of count instances described by p[0..count-1] (each as
  * fgpe_create would take it). */
@@ -218,11 +219,12 @@ int fgpe_batch_initial_guess(fgpe_batch* b);
 int fgpe_batch_upload(fgpe_batch* b, int32_t i, const double* phi);
 int fgpe_batch_download(fgpe_batch* b, int32_t i, double* phi);
 
-/* Advance every instance (converged or not) by nsteps steps in one launch;
- * afterwards each result holds NORMC, the observables and maxdiff. */
+/* Advance every instance (converged or not) by nsteps steps; afterwards each
+ * result holds NORMC of the last step, the observables and maxdiff (PHI_OLD
+ * = the state one step before the end). */
 int fgpe_batch_run(fgpe_batch* b, int64_t nsteps);
 
-/* Run to convergence: repeated launches of stp steps over the unconverged
+/* Run to convergence: repeated rounds of stp steps over the unconverged
  * instances until all have CONV <= tol or CONV_EN <= tol, or max_steps.
  * *converged receives the number of converged instances. */
 int fgpe_batch_solve(fgpe_batch* b, int32_t stp, double tol, int64_t max_steps, int32_t* converged);

@@ -16,11 +16,7 @@ from cases import rel_l2
 from fortress_amd import fgpe
 from fortress_amd.abi import FGPE_FP32, FGPE_FP64, make_input
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="batched launch disabled: its kernel faulted on the first device run "
-                            "(illegal memory access in the first k_batch launch) and the cause is not found yet"),
-]
+pytestmark = pytest.mark.gpu
 
 SWEEP = [  # n, GAMMAX, GAMMAY, A2 - A0, NATOMS, trap aspect
     (64, 0.3, 1.7, -4.0, 2000, 0.9),
@@ -145,21 +141,28 @@ def test_batch_tune_restores_state_and_keeps_results():
 def test_batch_solve_convergence_semantics():
     """solve() stops an instance at the first STP event with CONV <= TOL or
     CONV_EN <= TOL (CONV = max|phi - phi_old|/(STP 2 DT), phi_old one step
-    earlier; CONV_EN against the previous event's energy)."""
+    earlier; CONV_EN against the previous event's energy), and an instance
+    that is still running at max_steps has passed no event."""
     inps = [inst(64, 0.5, 0.5, 1.0, 2000, 1.0), inst(64, 1.5, 0.2, -3.0, 5000, 1.1),
             inst(128, 0.2, 1.0, 2.0, 20000, 0.9)]
     params = [fgpe.derive(i, FGPE_FP64)[0] for i in inps]
-    stp, tol = 50, 1e-4
+    # CONV of these instances (oracle): the first passes 2e-4 after about 1000
+    # steps, the other two stay above it through 3000
+    stp, tol, max_steps = 50, 2e-4, 3000
     with fgpe.Batch(params) as b:
         b.initial_guess()
         en0 = [b.result(i).en for i in range(3)]
-        nconv = b.solve(stp, tol, 5000)
+        nconv = b.solve(stp, tol, max_steps)
         res = [b.result(i) for i in range(3)]
-    assert nconv == 3
+    print("converged", nconv, "steps", [r.steps for r in res], flush=True)
+    assert nconv == sum(r.done for r in res) and nconv >= 1
     for i, r in enumerate(res):
-        assert r.done == 1 and r.steps % stp == 0 and r.steps > 0
-        assert r.conv <= tol or r.conv_en <= tol
-        # replay with the oracle: the same event must be the first to pass
+        assert r.steps % stp == 0 and r.steps > 0
+        if r.done:
+            assert r.conv <= tol or r.conv_en <= tol
+        else:
+            assert r.steps + stp > max_steps
+        # replay with the oracle: the first event that passes must be the last one run
         o = oracle_start(inps[i])
         en_old = en0[i]
         first = None
@@ -176,4 +179,4 @@ def test_batch_solve_convergence_semantics():
                 first = ev
                 break
         o.close()
-        assert first == r.steps // stp
+        assert first == (r.steps // stp if r.done else None)
