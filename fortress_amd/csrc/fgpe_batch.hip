@@ -50,9 +50,13 @@ __global__ __launch_bounds__(256) void kg_xpass(const GInst<T>* __restrict__ gi,
 template <typename T, int N, int Q, int TX>
 __global__ __launch_bounds__(TX *(N / Q)) void kg_kpass(const GInst<T>* __restrict__ gi, const int* __restrict__ list,
                                                         int nb, int scale) {
-  const GInst<T>& I = gi[list[blockIdx.x / nb]];
+  // blocks (instance, x block) laid out as pass C's grid: the two 64 B halves
+  // of a 128 B row go to workgroups 8 apart, on one XCD (kpass_coords)
+  int bx, by;
+  kpass_coords<TX * (int)sizeof(C_t<T>)>(bx, by);
+  const GInst<T>& I = gi[list[bx / nb]];
   const Geom g{N, N, 1, (long)N * N};
-  kpass_block<T, 1, 2, N, Q, TX>((int)(blockIdx.x % nb), 0, I.phi, g, I.tb, I.ph, scale);
+  kpass_block<T, 1, 2, N, Q, TX>(bx % nb, 0, I.phi, g, I.tb, I.ph, scale);
 }
 
 template <typename T>

@@ -421,9 +421,29 @@ __device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__
     });
   });
 }
+// Block coordinates of pass C.  When a block's rows are 64 B (half of a
+// 128 B line), the two blocks sharing each line are given workgroup ids 8
+// apart: workgroups go to the 8 XCDs round-robin (id mod 8), each XCD with its
+// own L2, so the pair runs on one XCD at the same time and the line is fetched
+// from memory once instead of once per half.
+template <int ROW_BYTES> __device__ __forceinline__ void kpass_coords(int& bx, int& by) {
+  const int nbx = gridDim.x, total = gridDim.x * gridDim.y;
+  if (ROW_BYTES != 64 || (nbx & 1) || (total & 15)) {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    return;
+  }
+  const int lin = blockIdx.x + blockIdx.y * nbx;
+  const int l = (lin & ~15) + ((lin & 7) << 1) + ((lin >> 3) & 1);
+  bx = l % nbx;
+  by = l / nbx;
+}
+
 template <typename T, int SPIN, int DIM, int N, int Q, int TX>
 __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
-  kpass_block<T, SPIN, DIM, N, Q, TX>(blockIdx.x, blockIdx.y, phi, g, tb, ph, scale);
+  int bx, by;
+  kpass_coords<TX * (int)sizeof(C_t<T>)>(bx, by);
+  kpass_block<T, SPIN, DIM, N, Q, TX>(bx, by, phi, g, tb, ph, scale);
 }
 
 // ------------------------------------------------------------------- 1D
