@@ -721,6 +721,16 @@ int fgpe_batch_initial_guess(fgpe_batch* b) {
   return 0;
 }
 
+int fgpe_batch_normalize(fgpe_batch* b) {
+  if (!b) return fail(FGPE_EINVAL, "null batch");
+  int rc = b->fp32 ? normt_t<float>(b) : normt_t<double>(b);
+  if (rc) return rc;
+  for (auto& r : b->res) r = fgpe_batch_result{};
+  if ((rc = advance(b, all_ids(b), 0, false, false))) return rc;
+  for (int i = 0; i < b->count; ++i) b->en_old[i] = b->res[i].en;
+  return 0;
+}
+
 int fgpe_batch_upload(fgpe_batch* b, int32_t i, const double* phi) {
   if (!b || !phi || i < 0 || i >= b->count) return fail(FGPE_EINVAL, "bad batch upload argument");
   const long n = (long)b->n[i] * b->n[i] * 3;

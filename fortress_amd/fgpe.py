@@ -40,6 +40,7 @@ EXPORTS = (
     "fgpe_batch_create",
     "fgpe_batch_destroy",
     "fgpe_batch_initial_guess",
+    "fgpe_batch_normalize",
     "fgpe_batch_upload",
     "fgpe_batch_download",
     "fgpe_batch_run",
@@ -102,6 +103,7 @@ def lib():
         L.fgpe_batch_destroy.argtypes = [vp]
         L.fgpe_batch_destroy.restype = None
         L.fgpe_batch_initial_guess.argtypes = [vp]
+        L.fgpe_batch_normalize.argtypes = [vp]
         L.fgpe_batch_upload.argtypes = [vp, i32, dp]
         L.fgpe_batch_download.argtypes = [vp, i32, dp]
         L.fgpe_batch_run.argtypes = [vp, i64]
@@ -270,6 +272,9 @@ class Batch:
 
     def initial_guess(self):
         _check(lib().fgpe_batch_initial_guess(self.h), "fgpe_batch_initial_guess")
+
+    def normalize(self):
+        _check(lib().fgpe_batch_normalize(self.h), "fgpe_batch_normalize")
 
     def upload(self, i, phi):
         a = np.ascontiguousarray(np.asarray(phi, dtype=np.complex128).reshape(self.shape(i)))

@@ -139,12 +139,28 @@ class Surrogate:
         self.net.close()
 
 
-def make_dataset(draws, tol=1e-7, stp=100):
-    """Coarse ground states (64 x 64 over the same box) for each draw."""
-    feats, targets, steps = [], [], []
-    for d in draws:
-        phi, k, _, _ = solve(problem(d, COARSE), stp=stp, tol=tol)
-        feats.append(features(d))
-        targets.append(to_vector(phi))
-        steps.append(k)
-    return np.array(feats), np.array(targets), steps
+def solve_batch(b, stp, tol, max_steps, chunk=20000, log=None):
+    """fgpe_batch_solve in step budgets of `chunk` (the solve resumes where it
+    stopped), calling log(steps_so_far, converged) after each budget."""
+    limit, nconv = 0, 0
+    while limit < max_steps:
+        limit = min(limit + chunk, max_steps)
+        nconv = b.solve(stp, tol, limit)
+        if log:
+            log(limit, nconv)
+        if nconv == b.count:
+            break
+    return nconv
+
+
+def make_dataset(draws, tol=1e-7, stp=100, max_steps=400000, log=None):
+    """Coarse ground states (64 x 64 over the same box) for each draw, solved
+    together as one batch (fgpe_batch_*: the same step and convergence test
+    as solve()).  Returns (features, targets, steps per draw)."""
+    params = [problem(d, COARSE) for d in draws]
+    with fgpe.Batch(params) as b:
+        b.initial_guess()
+        solve_batch(b, stp, tol, max_steps, log=log)
+        steps = [b.result(i).steps for i in range(len(draws))]
+        targets = [to_vector(b.download(i)) for i in range(len(draws))]
+    return np.array([features(d) for d in draws]), np.array(targets), steps

@@ -215,6 +215,12 @@ void fgpe_batch_destroy(fgpe_batch* b);
  * followed by fgpe_normalize), then the observables of the start state. */
 int fgpe_batch_initial_guess(fgpe_batch* b);
 
+/* NORMT of every instance (as fgpe_normalize) and the observables of that
+ * state, resetting the results: the start of a solve from uploaded states
+ * (SRC/cgpe2d.f90:401-407, the guess of INITIALIZE replaced by the
+ * uploaded states). */
+int fgpe_batch_normalize(fgpe_batch* b);
+
 /* Copy instance i in or out (layout of fgpe_upload/fgpe_download). */
 int fgpe_batch_upload(fgpe_batch* b, int32_t i, const double* phi);
 int fgpe_batch_download(fgpe_batch* b, int32_t i, double* phi);
