@@ -215,6 +215,8 @@ def batch_bench(args):
         inps.append(make_input(2, 1, (n, n), (0.1, 0.1), 1e-3, 87.0, (a0, a0 + rng.uniform(-5, 5), 0.0),
                                int(round(10 ** rng.uniform(3, 5))), (20.0, 20.0 * rng.uniform(0.8, 1.25), 400.0),
                                (rng.uniform(0, 2), rng.uniform(0, 2)), switch_im=1))
+    if args.batch_solve:
+        return batch_solve(args, inps)
     params = [fgpe.derive(i, FGPE_FP32)[0] for i in inps]
     sizes = [p.n[0] for p in params]
     pts = sum(n * n for n in sizes)
@@ -260,6 +262,60 @@ def batch_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def batch_solve(args, inps):
+    """The sweep solved to TOL (FP64, imaginary time, STP = 100, cgpe.x's
+    CONV/CONV_EN test, at most args.max_steps steps per instance) as one
+    batch, after the autotuner; the time the same instance-steps would take
+    one instance at a time comes from the unbatched FP64 rate measured on a
+    sample."""
+    from fortress_amd import fgpe
+    from fortress_amd import surrogate as S
+    from fortress_amd.abi import FGPE_FP64
+    for i in inps:
+        i.tol = args.batch_tol
+    params = [fgpe.derive(i, FGPE_FP64)[0] for i in inps]
+    sizes = [p.n[0] for p in params]
+    with fgpe.Batch(params) as b:
+        b.initial_guess()
+        ms_tune, chosen = b.tune(3)
+        t0 = time.perf_counter()
+        nconv = S.solve_batch(b, 100, args.batch_tol, args.max_steps,
+                              log=lambda k, n: print(json.dumps({"sweep": {"steps": k, "converged": n}}), flush=True))
+        sweep_s = time.perf_counter() - t0
+        steps = np.array([b.result(i).steps for i in range(len(params))])
+    by_size = {n: steps[[i for i, m in enumerate(sizes) if m == n]] for n in (64, 128, 192, 256)}
+    # unbatched FP64 rate per size class (192 has no single-instance path:
+    # its rate is taken as the mean of 128 and 256 per point)
+    rate = {}
+    for n in (64, 128, 256):
+        i = sizes.index(n)
+        with fgpe.Solver(params[i]) as s:
+            s.initial_guess()
+            s.normalize()
+            s.run(100)
+            s.sync()
+            t = time.perf_counter()
+            s.run(args.steps)
+            s.sync()
+            rate[n] = args.steps / (time.perf_counter() - t)
+    rate[192] = 2.0 / (1.0 / rate[128] + 1.0 / rate[256])
+    unbatched_s = sum(by_size[n].sum() / rate[n] for n in by_size)
+    line = {
+        "metric": "batched q2D spin-1 sweep solved to TOL: wall clock",
+        "value": round(sweep_s, 1), "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "converged": int(nconv), "instances": len(params), "total_instance_steps": int(steps.sum()),
+        "median_steps": int(np.median(steps)), "capped": int((steps >= args.max_steps).sum()),
+        "unbatched_estimate_s": round(unbatched_s, 1), "speedup_vs_unbatched": round(unbatched_s / sweep_s, 2),
+        "unbatched_steps_per_s": {str(n): round(r, 1) for n, r in rate.items()},
+        "dtype": "fp64", "data": "synthetic seeded parameter sweep, Gaussian initial guesses",
+        "config": {"workload": "%d q2D spin-1 imaginary-time instances, grids from {64,128,192,256}^2, DX=0.1, "
+                               "DT=0.001, TOL=%g, STP=100, max %d steps, autotuned schedules"
+                               % (len(params), args.batch_tol, args.max_steps),
+                   "tune_ms": [round(x, 4) for x in ms_tune], "chosen": list(chosen)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +342,9 @@ def main():
                     help="instead of the 3D propagator, the batched q2D parameter sweep")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--sample", type=int, default=16)
+    ap.add_argument("--batch-solve", action="store_true",
+                    help="with --batch: solve the sweep to --batch-tol in FP64 instead of timing steps")
+    ap.add_argument("--batch-tol", type=float, default=1e-8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
