@@ -78,6 +78,9 @@ template <typename T> int build_tables(fgpe_handle* h) {
   if ((rc = upload_vec(h, &h->kex, kex))) return rc;
   if ((rc = upload_vec(h, &h->key, key))) return rc;
   if ((rc = upload_vec(h, &h->kel, kel))) return rc;
+  if (h->dim == 2 &&
+      (rc = upload_vec(h, &h->soc2, soc2_table<T>(kx, ky, p.gamma[0], p.gamma[1], dt, imag, h->spin))))
+    return rc;
   std::vector<T> kxt(kx.begin(), kx.end()), kyt(ky.begin(), ky.end());
   if ((rc = upload_vec(h, &h->kx, kxt))) return rc;
   if ((rc = upload_vec(h, &h->ky, kyt))) return rc;

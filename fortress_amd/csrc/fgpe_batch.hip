@@ -312,6 +312,11 @@ template <typename T> int build_batch(fgpe_batch* b) {
         (rc = bupload(b, &t.vz, vz)) || (rc = bupload(b, &t.wx, wx)) || (rc = bupload(b, &t.wy, wy)) ||
         (rc = bupload(b, &t.wz, one)))
       return rc;
+    {
+      const fgpe::C_t<T>* st;
+      if ((rc = bupload(b, &st, soc2_table<T>(kx, ky, p.gamma[0], p.gamma[1], p.dt, p.imag, 1)))) return rc;
+      t.soc2 = st;
+    }
     t.key = t.kex;  // not read by the 2D passes
     t.twz = nullptr;
     t.cscale = I.cscale;

@@ -863,4 +863,28 @@ __host__ __device__ __forceinline__ void soc_factors(T kx, T ky, const Phys& p, 
   }
 }
 
+// e1 and dm1 of soc_factors from one entry of the 2D table (built on the host
+// in FP64 by soc2_table): t[0] = e1, t[L] packs the two factors of +-L --
+// real time (re, im) of exp(-i th) - 1 = (re, -im) for +L and its conjugate
+// for -L; imaginary time (expm1(-th), expm1(th)).  Forming the factors per
+// point in the 2D k pass (a square root, a division and a sincos per point)
+// held 250-290 VGPRs; the table read holds about 110.
+template <int SPIN, typename C>
+__host__ __device__ __forceinline__ void soc_from_table(const C* t, int imag, C& e1, C* dm1) {
+  using T = decltype(C::x);
+  e1 = t[0];
+  dm1[SPIN] = mk<C>((T)0, (T)0);
+#pragma unroll
+  for (int L = 1; L <= SPIN; ++L) {
+    const C d = t[L];
+    if (imag) {
+      dm1[SPIN + L] = mk<C>(d.x, (T)0);
+      dm1[SPIN - L] = mk<C>(d.y, (T)0);
+    } else {
+      dm1[SPIN + L] = mk<C>(d.x, -d.y);
+      dm1[SPIN - L] = mk<C>(d.x, d.y);
+    }
+  }
+}
+
 }  // namespace fgpe

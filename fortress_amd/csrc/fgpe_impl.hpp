@@ -136,6 +136,40 @@ template <typename T> R_t<T> ke_factor(double a, double dt, int imag) {
   return rot_from<T>(-2.0 * sh * sh, -std::sin(dt * a));
 }
 
+// 2D SOC factors per (ky, kx) for soc_from_table, in FP64 then rounded:
+// ALPHA, BETA and the factors of SRC/cgpe2d.f90 SOC (the 2D form of
+// SRC/cgpe3d.f90:904-1004): e1 = exp(-i BETA) = (GX kx - i GY ky)/|.| (1 when
+// ALPHA <= 1e-30), th = DT L ALPHA with ALPHA = 2 |(GX kx, GY ky)|.
+template <typename T>
+std::vector<fgpe::C_t<T>> soc2_table(const std::vector<double>& kx, const std::vector<double>& ky, double gx, double gy,
+                             double dt, int imag, int spin) {
+  const size_t nx = kx.size(), ny = ky.size();
+  std::vector<fgpe::C_t<T>> t(nx * ny * (spin + 1));
+  auto put = [&](size_t i, double a, double b) {
+    t[i].x = (T)a;
+    t[i].y = (T)b;
+  };
+  for (size_t y = 0; y < ny; ++y)
+    for (size_t x = 0; x < nx; ++x) {
+      const size_t o = (y * nx + x) * (spin + 1);
+      const double ax = gx * kx[x], ay = gy * ky[y], r = std::sqrt(ax * ax + ay * ay), alpha = 2.0 * r;
+      if (alpha <= 1e-30)
+        put(o, 1.0, 0.0);
+      else
+        put(o, ax / r, -ay / r);
+      for (int L = 1; L <= spin; ++L) {
+        const double th = dt * L * alpha;
+        if (imag) {
+          put(o + L, std::expm1(-th), std::expm1(th));
+        } else {
+          const double sh = std::sin(0.5 * th), ch = std::cos(0.5 * th);
+          put(o + L, -2.0 * sh * sh, 2.0 * sh * ch);
+        }
+      }
+    }
+  return t;
+}
+
 }  // namespace fgpe_impl
 
 struct fgpe_handle {
@@ -154,6 +188,7 @@ struct fgpe_handle {
   // tables (T precision unless noted; twiddles and KE factors rot_of<T>)
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;
   void *kex = nullptr, *key = nullptr, *kel = nullptr;  // KE factors (rot_of<T>)
+  void* soc2 = nullptr;  // 2D SOC factor table (soc2_table)
   void *kx = nullptr, *ky = nullptr;
   void *vx = nullptr, *vy = nullptr, *vz = nullptr;
   double *wx = nullptr, *wy = nullptr, *wz = nullptr;
@@ -203,6 +238,7 @@ template <typename T> inline fgpe::Tables<T> tables(fgpe_handle* h) {
   t.wy = h->wy;
   t.wz = h->wz;
   t.cscale = (const fgpe::C_t<T>*)h->cscale;
+  t.soc2 = (const fgpe::C_t<T>*)h->soc2;
   return t;
 }
 

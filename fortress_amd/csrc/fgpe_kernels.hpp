@@ -42,6 +42,7 @@ template <typename T> struct Tables {
   const double* wy;
   const double* wz;
   const C_t<T>* cscale;  // pending per-component renormalisation [MS]
+  const C_t<T>* soc2;    // 2D SOC factors per (ky, kx): [NY][NX][1 + SPIN] (soc2_table)
 };
 
 struct Geom {
@@ -394,7 +395,7 @@ __device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;
     const int pos = j + P * q;
-    if constexpr (DIM == 2) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[pos], ph, dt, e1, dm1);
+    if constexpr (DIM == 2) soc_from_table<SPIN, C>(tb.soc2 + ((long)pos * g.nx + x) * (SPIN + 1), ph.imag, e1, dm1);
     // line-axis KE factor, then the SOC change of the result
     C pt[MS], cr[MS];
     SFor<0, MS>::run([&](auto mi) {
