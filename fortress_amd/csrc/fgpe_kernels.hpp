@@ -187,7 +187,10 @@ __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ pa
 // real-space physics (1/N, SE, SP, NORMC partials), XF_FWD forward FFT last,
 // XF_SCALE multiply by the pending scale on load, XF_NORM (with XF_PHYS) write
 // the block's NORMC partials.
-template <typename T, int SPIN, int N, int Q>
+// FULL: the pass also applies the 1/N of BFT and the kx factor of KE (2D);
+// in 3D passes D and B apply them (they commute with the y transforms), so the
+// VALU-bound pass A carries less.
+template <typename T, int SPIN, int N, int Q, bool FULL = true>
 __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
                                             const Phys& ph, int flags, double* __restrict__ partials) {
   using C = C_t<T>;
@@ -211,7 +214,8 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
 
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
   if (flags & XF_FWD)
-    for (int i = threadIdx.x; i < N; i += blockDim.x) lds_kex[i] = tb.kex[i];
+    if (FULL)
+      for (int i = threadIdx.x; i < N; i += blockDim.x) lds_kex[i] = tb.kex[i];
 
   C v[MS][Q];
   const unsigned int voff = (unsigned int)((line * N + j) * (long)sizeof(C));
@@ -238,7 +242,7 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
       C pt[MS];
       SFor<0, MS>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
-        pt[m] = scalec(v[m][q], ph.inv_npts);
+        pt[m] = FULL ? scalec(v[m][q], ph.inv_npts) : v[m][q];
       });
       if constexpr (SPIN == 1)
         se1_point<C, T>(pt, ph, dt, sizeof(T) == 4 ? (T)0.05 : (T)1e-3);
@@ -287,28 +291,30 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
     SFor<0, MS>::run([&](auto mi) {
       constexpr int m = decltype(mi)::value;
       line_fft<N, Q, -1>(v[m], my, lds_tw, j);
-      SFor<0, Q>::run([&](auto qi) {
-        constexpr int q = decltype(qi)::value;
-        v[m][q] = rot1(v[m][q], lds_kex[j + P * q]);
-      });
+      if constexpr (FULL)
+        SFor<0, Q>::run([&](auto qi) {
+          constexpr int q = decltype(qi)::value;
+          v[m][q] = rot1(v[m][q], lds_kex[j + P * q]);
+        });
       store(mi);
     });
   } else {
     SFor<0, MS>::run(store);
   }
 }
-template <typename T, int SPIN, int N, int Q>
+template <typename T, int SPIN, int N, int Q, bool FULL>
 __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
                                                double* __restrict__ partials) {
-  xpass_block<T, SPIN, N, Q>(blockIdx.x, phi, g, tb, ph, flags, partials);
+  xpass_block<T, SPIN, N, Q, FULL>(blockIdx.x, phi, g, tb, ph, flags, partials);
 }
 
 // ------------------------------------------------------- pass B/D (y)
 // Lines along y for one component; a block owns TX consecutive x at one z.
-// dir -1 forward (then the ky factor of KE), +1 inverse; scale applies the
-// pending renormalisation.
+// dir -1 forward (then the ky and kx factors of KE), +1 inverse (then 1/N);
+// scale applies the pending renormalisation.  3D only.
 template <typename T, int N, int Q, int TX>
-__global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale) {
+__global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale,
+                                                       double inv_npts) {
   using C = C_t<T>;
   constexpr int P = N / Q;
   constexpr int LS = line_stride(N);
@@ -333,13 +339,20 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi,
   __syncthreads();
   if (dir < 0) {
     line_fft<N, Q, -1>(v, lds + l * LS, lds_tw, j);
-    // the ky factor of KE
+    // the ky and kx factors of KE (kx taken over from pass A: it is constant
+    // along the line and commutes with the y transform)
+    const R_t<T> fx = tb.kex[x];
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;
-      v[q] = rot1(v[q], tb.key[j + P * q]);
+      v[q] = rot1(rot1(v[q], tb.key[j + P * q]), fx);
     });
   } else {
     line_fft<N, Q, +1>(v, lds + l * LS, lds_tw, j);
+    // the 1/N of BFT, taken over from pass A (it commutes with the x transform)
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      v[q] = scalec(v[q], inv_npts);
+    });
   }
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;

@@ -26,7 +26,8 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
     int rc = ensure_partials(h, (size_t)blocks * MS);
     if (rc) return rc;
   }
-  auto kern = k_xpass<T, SPIN, N, Q>;
+  // 3D: passes B and D apply the kx factor and 1/N
+  auto kern = h->dim == 3 ? k_xpass<T, SPIN, N, Q, false> : k_xpass<T, SPIN, N, Q, true>;
   allow_lds(kern, lds);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
@@ -52,7 +53,7 @@ template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
   allow_lds(kern, lds);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(h->nx / TX, h->nz, h->ms), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h),
-                     tables<T>(h), dir, scale);
+                     tables<T>(h), dir, scale, 1.0 / (double)h->npts);
   HIPCHK(hipGetLastError());
   prof_end(h, FGPE_PASS_Y);
   return 0;
