@@ -34,7 +34,7 @@ PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
 # workload (256^3 spin-2 FP32), scaled by the counters' readings for a kernel
 # that reads and writes the spinor once: profiles/passes_3d_spin2_256_hbm_traffic.md
-TRAFFIC_256 = {"x": 1343350742, "y": 1342319658, "k": 1383498773}
+TRAFFIC_256 = {"x": 1343233715, "y": 1342329931, "k": 1373671637}
 
 
 def workload(n, imag, dt):
