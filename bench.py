@@ -135,7 +135,7 @@ def surrogate_bench(args):
     # the last tenth of the training draws is kept out of training as a
     # validation set (same distribution as the held-out draws)
     nv = max(len(train) // 10, 1)
-    sur = S.Surrogate(hidden=512, layers=4, max_batch=64, seed=args.seed)
+    sur = S.Surrogate(hidden=args.hidden, layers=4, max_batch=64, seed=args.seed)
     t1 = time.perf_counter()
     hist = sur.fit(feats[:-nv], targets[:-nv], epochs=args.epochs, batch=min(32, len(train) - nv), lr=args.lr,
                    lr_decay=args.lr_decay)
@@ -186,7 +186,8 @@ def surrogate_bench(args):
                                "draws: GAMMAX,GAMMAY~U[0,2], A0~U[99.8,103.8], A2-A0~U[-5,5] a_B, "
                                "NATOMS~logU[1e3,1e5], aspect~U[0.8,1.25], trap 20 Hz in plane, 400 Hz out"
                                % (args.n, args.tol),
-                   "network": "6-512x4-24576 (3 complex components on 64^2), BF16 MFMA, FP32 accumulation",
+                   "network": "6-%dx4-24576 (3 complex components on 64^2), BF16 MFMA, FP32 accumulation"
+                              % args.hidden,
                    "train_states": len(train), "train_tol": args.train_tol, "held_out": len(held),
                    "epochs": args.epochs, "lr": args.lr, "lr_decay": args.lr_decay, "max_steps": args.max_steps},
         "train_loss_last": hist[-1], "validation_loss": val,
@@ -335,6 +336,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--epochs", type=int, default=1200)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--hidden", type=int, default=512)
     ap.add_argument("--lr-decay", type=float, default=0.997)
     ap.add_argument("--max-steps", type=int, default=400000)
     ap.add_argument("--seed", type=int, default=2024)
