@@ -140,9 +140,13 @@ def surrogate_bench(args):
     hist = sur.fit(feats[:-nv], targets[:-nv], epochs=args.epochs, batch=min(32, len(train) - nv), lr=args.lr,
                    lr_decay=args.lr_decay)
     train_s = time.perf_counter() - t1
-    val = float(np.mean((sur.net.forward(feats[-nv:]) - targets[-nv:] / sur.scale) ** 2))
+    pred = sur.net.forward(feats[-nv:])
+    val = float(np.mean((pred - targets[-nv:] / sur.scale) ** 2))
+    # spread of the predictions across the validation draws against that of
+    # the targets (a network that ignores its inputs has spread 0)
+    spread = float(np.mean(pred.std(axis=0)) / np.mean((targets[-nv:] / sur.scale).std(axis=0)))
     print(json.dumps({"train_loss_first": hist[0], "train_loss_last": hist[-1], "validation_loss": val,
-                      "train_s": round(train_s, 1)}), flush=True)
+                      "prediction_spread": spread, "train_s": round(train_s, 1)}), flush=True)
     params = [S.problem(d, args.n) for d in held]
     with fgpe.Batch(params) as b:
         b.initial_guess()
@@ -190,7 +194,7 @@ def surrogate_bench(args):
                               % args.hidden,
                    "train_states": len(train), "train_tol": args.train_tol, "held_out": len(held),
                    "epochs": args.epochs, "lr": args.lr, "lr_decay": args.lr_decay, "max_steps": args.max_steps},
-        "train_loss_last": hist[-1], "validation_loss": val,
+        "train_loss_last": hist[-1], "validation_loss": val, "prediction_spread": spread,
         "data_generation_s": round(gen_s, 1), "training_s": round(train_s, 1),
         "dtype": "fp64 solves, bf16/fp32 network",
     }

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(70, 130, 50), (64, 512, 4096), (96, 1000, 6)])
+@pytest.mark.parametrize("m,n,k", [(70, 130, 50), (64, 512, 4096), (96, 1000, 6), (32, 512, 24576)])
 def test_gemm_matches_numpy(ta, tb, m, n, k):
     rng = np.random.default_rng(m + n + k + 2 * ta + tb)
     a = rng.standard_normal((m, k)).astype(np.float32)
@@ -26,8 +26,8 @@ def test_gemm_matches_numpy(ta, tb, m, n, k):
     assert err.max() < 2e-6
 
 
-def test_gradients_match_cpu_restatement():
-    dims = [6, 64, 64, 96]
+@pytest.mark.parametrize("dims", [[6, 64, 64, 96], [6, 64, 64, 24576]])
+def test_gradients_match_cpu_restatement(dims):
     rng = np.random.default_rng(7)
     x = rng.uniform(-1, 1, (16, dims[0])).astype(np.float32)
     t = rng.standard_normal((16, dims[-1])).astype(np.float32) * 0.5
