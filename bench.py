@@ -140,11 +140,13 @@ def surrogate_bench(args):
     hist = sur.fit(feats[:-nv], targets[:-nv], epochs=args.epochs, batch=min(32, len(train) - nv), lr=args.lr,
                    lr_decay=args.lr_decay)
     train_s = time.perf_counter() - t1
-    pred = sur.net.forward(feats[-nv:])
-    val = float(np.mean((pred - targets[-nv:] / sur.scale) ** 2))
-    # spread of the predictions across the validation draws against that of
-    # the targets (a network that ignores its inputs has spread 0)
-    spread = float(np.mean(pred.std(axis=0)) / np.mean((targets[-nv:] / sur.scale).std(axis=0)))
+    pred = sur.predict_vectors(feats[-nv:])
+    tv = targets[-nv:]
+    # validation error relative to the targets' spread (1 = no better than
+    # the mean state), and the spread of the predictions across the
+    # validation draws against that of the targets (0: the inputs are ignored)
+    val = float(np.mean((pred - tv) ** 2) / np.mean((tv - tv.mean(axis=0)) ** 2))
+    spread = float(np.mean(pred.std(axis=0)) / np.mean(tv.std(axis=0)))
     print(json.dumps({"train_loss_first": hist[0], "train_loss_last": hist[-1], "validation_loss": val,
                       "prediction_spread": spread, "train_s": round(train_s, 1)}), flush=True)
     params = [S.problem(d, args.n) for d in held]

@@ -122,15 +122,25 @@ class Surrogate:
     def __init__(self, hidden=512, layers=4, max_batch=64, seed=0):
         dims = [6] + [hidden] * layers + [6 * COARSE * COARSE]
         self.net = MLP(dims, max_batch=max_batch, seed=seed)
-        self.scale = 1.0
+        self.mean, self.sd = 0.0, 1.0
 
     def fit(self, feats, targets, epochs, batch, lr, seed=0, lr_decay=0.97):
-        self.scale = float(np.abs(targets).max()) or 1.0
-        self.net.set_dataset(feats, targets / self.scale)
+        """Trains on standardised targets: per output, minus its mean over the
+        training set, over its spread (floored at 1e-3 of the largest spread),
+        so the loss weighs how the state changes with the parameters rather
+        than the state all draws share."""
+        t = np.asarray(targets, dtype=np.float64)
+        self.mean = t.mean(axis=0)
+        sd = t.std(axis=0)
+        self.sd = np.maximum(sd, 1e-3 * sd.max()) if sd.max() > 0 else np.ones_like(sd)
+        self.net.set_dataset(feats, ((t - self.mean) / self.sd).astype(np.float32))
         return self.net.train(epochs, batch, lr, seed=seed, lr_decay=lr_decay)
 
+    def predict_vectors(self, feats):
+        return self.net.forward(feats) * self.sd + self.mean
+
     def predict(self, d):
-        return from_vector(self.net.forward(features(d)[None])[0] * self.scale)
+        return from_vector(self.predict_vectors(features(d)[None])[0])
 
     def guess(self, d, n):
         return upsample(self.predict(d), n)
