@@ -171,7 +171,7 @@ def surrogate_bench(args):
     sp = np.array([r.steps for r in rp], float)
     ss = np.array([r.steps for r in rs], float)
     gap = np.array([abs(s.en / p.en - 1) for s, p in zip(rs, rp)])
-    both = np.array([p.done and s.done for p, s in zip(rp, rs)])
+    both = np.array([bool(p.done) and bool(s.done) for p, s in zip(rp, rs)], dtype=bool)
     same = both & (gap <= 1e-8)
     for i in range(len(held)):
         print(json.dumps({"draw": i, "plain_steps": int(sp[i]), "surrogate_steps": int(ss[i]),
