@@ -510,3 +510,90 @@ def test_projection_keeps_norm_and_magnetisation(dim, spin, mag):
     assert abs(nrm.sum() - 1.0) < 1e-12
     tol = 1e-12 if spin == 1 else 1e-8
     assert abs(np.dot(m, nrm) - mag) < tol, (np.dot(m, nrm), mag)
+
+
+# ------------------------------------------------------------ initial guess
+GUESS_CASES = [
+    # dim, spin, OPTION_FPC, MAG, n, d, natoms, nu, profile of the run's branch
+    (3, 1, 1, 0.3, 96, 0.4, 3e7, (20.0, 25.0, 30.0), "tf"),
+    (3, 1, 1, -0.6, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (3, 2, 1, -0.8, 80, 0.4, 1e7, (20.0, 25.0, 30.0), "tf"),
+    (3, 2, 1, 0.5, 48, 0.4, 1e5, (20.0, 25.0, 30.0), "gauss"),
+    (2, 2, 1, 1.2, 256, 0.15, 1e6, (20.0, 25.0, 400.0), "tf"),
+    (2, 1, 1, 0.2, 128, 0.15, 1e3, (20.0, 25.0, 400.0), "gauss"),
+    (1, 2, 1, 0.7, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+    (3, 1, 2, 0.4, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (2, 2, 2, -0.6, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
+    (3, 2, 3, 0.25, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (1, 2, 3, -0.5, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+    (3, 1, 4, 0.0, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (3, 2, 4, 0.0, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (2, 2, 4, 0.0, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
+    (1, 1, 4, 0.0, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+]
+
+
+@pytest.mark.parametrize("dim,spin,fpc,mag,n,d,natoms,nu,prof", GUESS_CASES)
+def test_initial_guess_spinor_and_profile(dim, spin, fpc, mag, n, d, natoms, nu, prof):
+    """INITIALIZE's guesses (SRC/cgpe3d.f90:754-833, SRC/cgpe2d.f90:697-771,
+    SRC/cgpe1d.f90:707-786) are phi_m(r) = c_m f(r), real, with
+
+    * spinor c: ferromagnetic (OPTION_FPC 1) = the fully polarised state
+      |m=F> rotated about y by cos(beta) = MAG/F, i.e. the Wigner column
+      d^F_{m,F}(beta) from expm(-i beta F_y); antiferromagnetic / polar /
+      cyclic (2, 3) = unit spinors on {+1,-1}, {+2,-2}, {+2,-1} whose
+      magnetisation sum_m m c_m^2 is MAG (two magnitudes, two conditions);
+      option 4 = equal components;
+    * profile f: the Gaussian exp(-sum alpha_d x_d^2 / 2) / pi^(dim/4), whose
+      integral of f^2 is 1/sqrt(prod alpha); the Thomas-Fermi branch
+      (mu - sum alpha_d^2 x_d^2)_+ / g with mu^(5/2) = 15 g / (8 pi) in 3D and
+      mu^2 = 2 g / pi in 2D, whose integral is 1/prod alpha in both.
+
+    Option 4 keeps the reference's quirks: in 3D spin-1 the Gaussian is the
+    isotropic exp(-r^2/2) / pi^(3/4) (integral 1), and the 1D/2D spin-2
+    components carry sqrt(0.2).  A wrong Wigner entry, a swapped component,
+    a lost alpha power or a wrong TF exponent fails one of these."""
+    inp = case(dim, spin, (n,) * dim, (d,) * dim, 1e-3, (0.0,) * dim, natoms=natoms, nu=nu, option_fpc=fpc,
+               mag=mag)
+    p, _ = O.derive(inp)
+    o = O.Oracle(p)
+    o.initial_guess()
+    phi = o.get_phi()
+    o.close()
+    assert np.abs(phi.imag).max() == 0.0
+    phi = phi.real
+    dens = np.sum(phi ** 2, axis=0)
+    q = np.unravel_index(np.argmax(dens), dens.shape)
+    c = phi[(slice(None),) + q]
+    c = c / np.linalg.norm(c)
+    f = np.sqrt(dens)
+    assert np.allclose(phi, c[:, None, None, None] * f[None], rtol=0, atol=1e-14 * f.max())
+
+    ms = 2 * spin + 1
+    m = np.arange(spin, -spin - 1, -1)
+    if fpc == 1:
+        beta = np.arccos(mag / spin)
+        want = np.abs(sl.expm(-1j * beta * SPIN_MATS[spin][1])[:, 0])
+        assert np.allclose(c, want, rtol=0, atol=1e-13), (c, want)
+    elif fpc in (2, 3):
+        support = {(1, 2): (1, -1), (2, 2): (2, -2), (2, 3): (2, -1)}[(spin, fpc)]
+        off = [i for i in range(ms) if m[i] not in support]
+        assert np.all(c[off] == 0.0) and np.all(c[[list(m).index(s) for s in support]] > 0.0)
+        assert abs(np.dot(m, c ** 2) - mag) < 1e-13
+    else:
+        assert np.allclose(c, np.full(ms, 1.0 / np.sqrt(ms)), rtol=0, atol=1e-14)
+
+    alpha = np.array(list(p.alpha)[:dim])
+    total = dens.sum() * d ** dim
+    if prof == "tf":
+        assert abs(total * np.prod(alpha) - 1.0) < 1e-4, total * np.prod(alpha)
+    else:
+        if fpc != 4:
+            scale = 1.0
+        elif dim == 3 and spin == 1:
+            scale, alpha = 3.0, np.ones(3)
+        elif spin == 2 and dim < 3:
+            scale = ms * 0.2
+        else:
+            scale = float(ms)
+        assert abs(total * np.sqrt(np.prod(alpha)) / scale - 1.0) < 1e-10, total * np.sqrt(np.prod(alpha)) / scale
