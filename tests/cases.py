@@ -30,3 +30,25 @@ def noisy(phi, seed, amp=0.01):
 
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+# Initial-guess cases over OPTION_FPC 1-4, with the branch (Thomas-Fermi or
+# Gaussian profile) that the reference takes for each.
+GUESS_CASES = [
+    # dim, spin, OPTION_FPC, MAG, n, d, natoms, nu, profile of the run's branch
+    (3, 1, 1, 0.3, 128, 0.3, 3e7, (20.0, 25.0, 30.0), "tf"),
+    (3, 1, 1, -0.6, 64, 0.3, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (3, 2, 1, -0.8, 128, 0.25, 1e7, (20.0, 25.0, 30.0), "tf"),
+    (3, 2, 1, 0.5, 64, 0.3, 1e5, (20.0, 25.0, 30.0), "gauss"),
+    (2, 2, 1, 1.2, 256, 0.15, 1e6, (20.0, 25.0, 400.0), "tf"),
+    (2, 1, 1, 0.2, 128, 0.15, 1e3, (20.0, 25.0, 400.0), "gauss"),
+    (1, 2, 1, 0.7, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+    (3, 1, 2, 0.4, 64, 0.3, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (2, 2, 2, -0.6, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
+    (3, 2, 3, 0.25, 64, 0.3, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (1, 2, 3, -0.5, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+    (3, 1, 4, 0.0, 64, 0.3, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (3, 2, 4, 0.0, 64, 0.3, 1e4, (20.0, 25.0, 30.0), "gauss"),
+    (2, 2, 4, 0.0, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
+    (1, 1, 4, 0.0, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+]

@@ -25,7 +25,7 @@ import scipy.linalg as sl
 from scipy.integrate import solve_ivp
 
 import oracle_ctypes as O
-from cases import NA_SPIN2, case
+from cases import GUESS_CASES, NA_SPIN2, case
 from fortress_amd.inputs import read_input
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -513,24 +513,6 @@ def test_projection_keeps_norm_and_magnetisation(dim, spin, mag):
 
 
 # ------------------------------------------------------------ initial guess
-GUESS_CASES = [
-    # dim, spin, OPTION_FPC, MAG, n, d, natoms, nu, profile of the run's branch
-    (3, 1, 1, 0.3, 96, 0.4, 3e7, (20.0, 25.0, 30.0), "tf"),
-    (3, 1, 1, -0.6, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
-    (3, 2, 1, -0.8, 80, 0.4, 1e7, (20.0, 25.0, 30.0), "tf"),
-    (3, 2, 1, 0.5, 48, 0.4, 1e5, (20.0, 25.0, 30.0), "gauss"),
-    (2, 2, 1, 1.2, 256, 0.15, 1e6, (20.0, 25.0, 400.0), "tf"),
-    (2, 1, 1, 0.2, 128, 0.15, 1e3, (20.0, 25.0, 400.0), "gauss"),
-    (1, 2, 1, 0.7, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
-    (3, 1, 2, 0.4, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
-    (2, 2, 2, -0.6, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
-    (3, 2, 3, 0.25, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
-    (1, 2, 3, -0.5, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
-    (3, 1, 4, 0.0, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
-    (3, 2, 4, 0.0, 48, 0.4, 1e4, (20.0, 25.0, 30.0), "gauss"),
-    (2, 2, 4, 0.0, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
-    (1, 1, 4, 0.0, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
-]
 
 
 @pytest.mark.parametrize("dim,spin,fpc,mag,n,d,natoms,nu,prof", GUESS_CASES)
