@@ -116,9 +116,21 @@ class MLP:
             out[s:s + xb.shape[0]] = yb
         return out
 
-    def gradients(self, x, t):
+    def _rows(self, x, t):
+        """x as [rows][in] and t as [rows][out], or FgpeError: the library
+        copies rows * width values from each host buffer."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != self.dims[0]:
+            raise FgpeError("inputs must have shape (rows, %d), got %s" % (self.dims[0], x.shape))
+        if t.shape != (x.shape[0], self.dims[-1]):
+            raise FgpeError("targets must have shape (%d, %d), got %s" % (x.shape[0], self.dims[-1], t.shape))
+        return x, t
+
+    def gradients(self, x, t):
+        x, t = self._rows(x, t)
+        if x.shape[0] > self.max_batch:
+            raise FgpeError("batch of %d rows exceeds max_batch %d" % (x.shape[0], self.max_batch))
         g = np.empty(self.nparams, dtype=np.float32)
         loss = C.c_double()
         _check(lib().fgpe_mlp_gradients(self.h, _fp(x), _fp(t), x.shape[0], _fp(g), C.byref(loss)),
@@ -126,8 +138,7 @@ class MLP:
         return g, loss.value
 
     def set_dataset(self, x, y):
-        self._x = np.ascontiguousarray(x, dtype=np.float32)
-        self._y = np.ascontiguousarray(y, dtype=np.float32)
+        self._x, self._y = self._rows(x, y)
         _check(lib().fgpe_mlp_set_dataset(self.h, _fp(self._x), _fp(self._y), self._x.shape[0]),
                "fgpe_mlp_set_dataset")
         self.rows = self._x.shape[0]

@@ -353,6 +353,17 @@ void or_get_phi(const or_state* s, double* phi) {
   }
 }
 
+void or_get_phif(const or_state* s, double* phif) {
+  for (long q = 0; q < s->ms * s->npts; q++) {
+    phif[2 * q] = creal(s->phif[q]);
+    phif[2 * q + 1] = cimag(s->phif[q]);
+  }
+}
+
+void or_set_phif(or_state* s, const double* phif) {
+  for (long q = 0; q < s->ms * s->npts; q++) s->phif[q] = phif[2 * q] + I * phif[2 * q + 1];
+}
+
 /* --------------------------------------------------------- initial guess
  * INITIALIZE's guess branch: 3D SRC/cgpe3d.f90:753-833, 2D
  * SRC/cgpe2d.f90:697-771, 1D SRC/cgpe1d.f90:707-786.  The quirks are kept:

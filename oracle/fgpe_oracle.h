@@ -27,6 +27,9 @@ void or_destroy(or_state* s);
 void or_initial_guess(or_state* s);
 void or_set_phi(or_state* s, const double* phi);
 void or_get_phi(const or_state* s, double* phi);
+/* the transform buffer PHIF (k space between FFT and BFT), same layout as phi */
+void or_get_phif(const or_state* s, double* phif);
+void or_set_phif(or_state* s, const double* phif);
 double or_normt(or_state* s);
 int or_run(or_state* s, long nsteps);
 void or_last_norms(const or_state* s, double* norm);

@@ -51,4 +51,11 @@ GUESS_CASES = [
     (3, 2, 4, 0.0, 64, 0.3, 1e4, (20.0, 25.0, 30.0), "gauss"),
     (2, 2, 4, 0.0, 128, 0.15, 1e4, (20.0, 25.0, 400.0), "gauss"),
     (1, 1, 4, 0.0, 512, 0.05, 1e2, (20.0, 400.0, 400.0), "gauss"),
+    # 1D Thomas-Fermi: the reference raises to 0.67 (not 2/3), see tf1d below
+    (1, 1, 1, 0.3, 512, 0.05, 1e4, (20.0, 400.0, 400.0), "tf1d"),
+    (1, 2, 1, -0.4, 512, 0.05, 1e3, (20.0, 400.0, 400.0), "tf1d"),
+    # 2D spin-1 Thomas-Fermi: the support test compares V with the 8th power
+    # of the TF scale, so the reference takes sqrt of a negative number
+    # outside the TF radius (NaN there)
+    (2, 1, 1, 0.2, 256, 0.15, 1e6, (20.0, 25.0, 400.0), "tf_nan"),
 ]

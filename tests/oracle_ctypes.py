@@ -30,6 +30,8 @@ def lib():
         L.or_initial_guess.argtypes = [vp]
         L.or_set_phi.argtypes = [vp, dp]
         L.or_get_phi.argtypes = [vp, dp]
+        L.or_get_phif.argtypes = [vp, dp]
+        L.or_set_phif.argtypes = [vp, dp]
         L.or_normt.argtypes = [vp]
         L.or_normt.restype = C.c_double
         L.or_run.argtypes = [vp, C.c_long]
@@ -115,6 +117,15 @@ class Oracle:
         a = np.empty(self.shape, dtype=np.complex128)
         lib().or_get_phi(self.h, _dp(a.view(np.float64)))
         return a
+
+    def get_phif(self):
+        a = np.empty(self.shape, dtype=np.complex128)
+        lib().or_get_phif(self.h, _dp(a.view(np.float64)))
+        return a
+
+    def set_phif(self, phif):
+        a = np.ascontiguousarray(phif, dtype=np.complex128).reshape(self.shape)
+        lib().or_set_phif(self.h, _dp(a.view(np.float64)))
 
     def normt(self):
         return lib().or_normt(self.h)

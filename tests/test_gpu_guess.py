@@ -5,7 +5,9 @@ SRC/cgpe1d.f90:707-786) is evaluated pointwise on the GPU by
 fgpe_initial_guess; the oracle's guess is pinned by closed forms in
 test_oracle_pins.py.  FP64 handles agree to rounding (libm and the device
 math library may differ in the last bits of pow/exp); FP32 handles hold the
-guess rounded to float.
+guess rounded to float.  The 2D spin-1 Thomas-Fermi branch keeps the
+reference's NaN outside the TF radius (SRC/cgpe2d.f90:697-771): the device
+must put its NaNs at the same points.
 """
 import numpy as np
 import pytest
@@ -33,6 +35,12 @@ def test_device_guess_matches_oracle(dim, spin, fpc, mag, n, d, natoms, nu, prof
         with fgpe.Solver(pg) as s:
             s.initial_guess()
             got = s.download()
+        if prof == "tf_nan":
+            # the reference's sqrt of a negative number outside the TF radius:
+            # NaN at the same points on both sides, agreement elsewhere
+            bad = np.isnan(ref)
+            assert bad.any() and np.array_equal(np.isnan(got), bad)
+            got, ref = np.where(bad, 0.0, got), np.where(bad, 0.0, ref)
         err = rel_l2(got, ref)
         print("  rel L2", err, flush=True)
         assert np.isfinite(got).all() and err < tol

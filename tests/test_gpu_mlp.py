@@ -26,7 +26,7 @@ def test_gemm_matches_numpy(ta, tb, m, n, k):
     assert err.max() < 2e-6
 
 
-@pytest.mark.parametrize("dims", [[6, 64, 64, 96], [6, 64, 64, 24576]])
+@pytest.mark.parametrize("dims", [[6, 64, 64, 96], [6, 64, 64, 24576], [6, 512, 512, 512, 512, 24576]])
 def test_gradients_match_cpu_restatement(dims):
     rng = np.random.default_rng(7)
     x = rng.uniform(-1, 1, (16, dims[0])).astype(np.float32)

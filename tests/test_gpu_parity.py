@@ -54,6 +54,10 @@ CASES = {
                             nu=(20, 20, 20)), 10),
     "3d_s2_nosoc_newton": (case(3, 2, (32, 32, 32), (0.4, 0.4, 0.4), 5e-4, (0.0, 0.0, 0.0), mag=0.4,
                                 natoms=100000, nu=(20, 20, 20)), 10),
+    "3d_s1_nosoc_sigma": (case(3, 1, (32, 32, 32), (0.4, 0.4, 0.4), 5e-4, (0.0, 0.0, 0.0), mag=0.25,
+                               natoms=100000, nu=(20, 20, 20)), 10),
+    "2d_s2_nosoc_newton": (case(2, 2, (64, 64), (0.1, 0.1), 5e-4, (0.0, 0.0), mag=0.6, natoms=100000,
+                                nu=(20, 20, 400)), 20),
 }
 
 

@@ -30,3 +30,22 @@ def test_vector_roundtrip_and_features():
     assert f.shape == (6,) and np.all(np.abs(f) <= 1.01)
     p = S.problem(d, 256)
     assert p.n[0] == 256 and abs(p.d[0] - 0.1) < 1e-15 and abs(p.alpha[1] - d["aspect"]) < 1e-12
+
+
+def test_mlp_host_rejects_mismatched_rows():
+    """The library copies rows * width values from the host buffers, so the
+    wrapper checks the shapes before any call (no GPU needed here)."""
+    import pytest
+
+    from fortress_amd.fgpe import FgpeError
+    from fortress_amd.mlp import MLP
+
+    net = object.__new__(MLP)
+    net.dims, net.max_batch = [6, 8, 10], 4
+    x, t = np.zeros((3, 6)), np.zeros((3, 10))
+    assert net._rows(x, t)[0].shape == (3, 6)
+    for bad_x, bad_t in ((np.zeros((3, 5)), t), (x, np.zeros((2, 10))), (x, np.zeros((3, 9))), (np.zeros(18), t)):
+        with pytest.raises(FgpeError):
+            net._rows(bad_x, bad_t)
+    with pytest.raises(FgpeError):
+        net.gradients(np.zeros((5, 6)), np.zeros((5, 10)))
