@@ -35,8 +35,18 @@ def test_gradients_match_cpu_restatement(dims):
         p = net.get_params()
         g, loss = net.gradients(x, t)
         y = net.forward(x)
-    hs, _ = M.forward(p, dims, x)
-    assert np.allclose(y, hs[-1], rtol=1e-5, atol=1e-5)
+    hs, hb = M.forward(p, dims, x)
+    # FP32 accumulation (GPU) and FP64 (restatement) can round a hidden value
+    # to different BF16 neighbours; each such flip moves an output by at most
+    # one BF16 spacing (2^-8 relative) of |h| |W|.  So: relative L2 1e-5
+    # overall and at most 2^-7 of sum_j |h_j| |W_jk| at any output.
+    w_last = M.unpack(p, dims)[-1][0]
+    scale = np.abs(hb[-2]) @ np.abs(w_last)
+    dy = np.abs(y - hs[-1])
+    yerr = np.linalg.norm(y - hs[-1]) / np.linalg.norm(hs[-1])
+    print("forward rel l2", yerr, "max |dy|", dy.max(), "max |dy|/scale", (dy / scale).max(), flush=True)
+    assert yerr < 1e-5
+    assert np.all(dy <= 2.0**-7 * scale + 1e-7)
     ref_loss, ref_g = M.loss_and_grad(p, dims, x, t)
     print("loss", loss, ref_loss, "grad rel l2", np.linalg.norm(g - ref_g) / np.linalg.norm(ref_g), flush=True)
     assert abs(loss - ref_loss) <= 1e-5 * ref_loss
