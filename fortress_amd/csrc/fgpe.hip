@@ -7,9 +7,54 @@
 // launches HIP kernels and fails with an error when no device is usable.
 #include "fgpe_impl.hpp"
 
+#include <mutex>
+
+// Scratch of fgpe_run_noalloc: the NORMC block partials of pass A, held in a
+// fixed device array instead of memory the library allocates.  The largest
+// supported grid needs 655,360 doubles (512^3 spin-2 FP32: 131,072 x-pass
+// blocks x 5 components), so 2^20 doubles (8 MiB per device) covers every
+// grid fgpe_create accepts.  Each device holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (scratch_ptr below), never through the
+// array's name.
+static constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
 namespace fgpe_impl {
 
 thread_local std::string g_err;
+
+// per-device address of `scratch`, looked up once per device by fgpe_create
+// with that device current
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MAX_DEVICES];
+static bool g_scratch_done[MAX_DEVICES];
+static std::string g_scratch_err[MAX_DEVICES];
+
+static void scratch_setup(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[dev]) return;
+  g_scratch_done[dev] = true;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratch_err[dev] = hipGetErrorString(e);
+    return;
+  }
+  g_scratch_ptr[dev] = (double*)p;
+}
+
+static double* scratch_ptr(int dev, std::string* why) {
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    *why = "device index out of range";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) *why = g_scratch_done[dev] ? g_scratch_err[dev] : "not set up on this device";
+  return g_scratch_ptr[dev];
+}
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -146,6 +191,9 @@ Geom geom(fgpe_handle* h) { return Geom{h->nx, h->ny, h->nz, h->npts}; }
 
 int ensure_partials(fgpe_handle* h, size_t n) {
   if (h->partials_n >= n) return 0;
+  if (h->noalloc)
+    return fail(FGPE_EINVAL, "the no-allocation run needs %zu doubles of scratch; its limit is %zu", n,
+                h->partials_n);
   void* p;
   int rc = dalloc(h, &p, n * sizeof(double));
   if (rc) return rc;
@@ -298,6 +346,7 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
     return code;
   };
   if (hipGetDevice(&h->dev) != hipSuccess) return bail(fail(FGPE_EDEVICE, "hipGetDevice failed"));
+  scratch_setup(h->dev);  // once per device, this device current
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, h->dev) == hipSuccess) h->lds_max = prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
@@ -399,6 +448,31 @@ int fgpe_run(fgpe_handle* h, int64_t nsteps) {
   if (err)
     return fail(FGPE_ENEWTON, "Newton-Raphson could not calculate the roots within 1e-6 tolerance");
   return 0;
+}
+
+int fgpe_run_noalloc(fgpe_handle* h, int64_t nsteps) {
+  if (!h) return fail(FGPE_EINVAL, "null handle");
+  if (nsteps < 0) return fail(FGPE_EINVAL, "negative step count");
+  std::string why;
+  double* ptr = scratch_ptr(h->dev, &why);
+  if (!ptr) return fail(FGPE_EDEVICE, "no-allocation scratch unavailable on device %d: %s", h->dev, why.c_str());
+  // pass A writes MS partials per block (q1D keeps its norms inside its one block)
+  const size_t need = h->dim == 1 ? 0
+                      : (size_t)h->ms * (size_t)(h->fp32 ? x_pass_blocks<float>(h->nx, (long)h->ny * h->nz)
+                                                         : x_pass_blocks<double>(h->nx, (long)h->ny * h->nz));
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    return fail(FGPE_EINVAL, "grid %dx%dx%d needs %zu doubles of scratch; the no-allocation limit is %zu", h->nx,
+                h->ny, h->nz, need, NOALLOC_SCRATCH_DOUBLES);
+  double* own = h->partials;
+  const size_t own_n = h->partials_n;
+  h->partials = ptr;
+  h->partials_n = NOALLOC_SCRATCH_DOUBLES;
+  h->noalloc = true;
+  const int rc = fgpe_run(h, nsteps);
+  h->noalloc = false;
+  h->partials = own;
+  h->partials_n = own_n;
+  return rc;
 }
 
 int fgpe_last_norms(fgpe_handle* h, double* norm) {

@@ -55,6 +55,15 @@ template <> struct Cfg<double> {
   static constexpr int TXK = 8;
 };
 
+// Blocks of one pass-A launch: P = NX/QX threads per x line, lines packed into
+// blocks of at least 256 threads.  Each block writes MS NORMC partials.
+template <typename T> inline long x_pass_blocks(int nx, long nlines) {
+  const int p = nx / Cfg<T>::QX;
+  const int threads = p >= 256 ? p : 256;
+  const int lpw = threads / p;
+  return (nlines + lpw - 1) / lpw;
+}
+
 // ---------------------------------------------------- host-side tables
 // wave numbers in FFT order with positive Nyquist (SRC/cgpe3d.f90:702-726)
 inline std::vector<double> kgrid(int n, double d) {
@@ -201,6 +210,7 @@ struct fgpe_handle {
   int* derr = nullptr;
   double* partials = nullptr;    // reduction scratch
   size_t partials_n = 0;
+  bool noalloc = false;          // partials is the no-allocation scratch: never grow it
   double last_ms = 0;
   bool timed = false;
   // per-kernel timing (fgpe_pass_times): events recorded around launches

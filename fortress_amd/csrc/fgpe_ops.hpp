@@ -18,8 +18,7 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const int threads = P >= 256 ? P : 256;
   const int lpw = threads / P;
-  const long nlines = (long)h->ny * h->nz;
-  const int blocks = (int)((nlines + lpw - 1) / lpw);
+  const int blocks = (int)x_pass_blocks<T>(N, (long)h->ny * h->nz);
   const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
   if (flags & XF_PHYS) {

@@ -27,6 +27,7 @@ EXPORTS = (
     "fgpe_download",
     "fgpe_normalize",
     "fgpe_run",
+    "fgpe_run_noalloc",
     "fgpe_last_norms",
     "fgpe_normc",
     "fgpe_observables",
@@ -91,6 +92,7 @@ def lib():
         L.fgpe_download.argtypes = [vp, dp]
         L.fgpe_normalize.argtypes = [vp, dp]
         L.fgpe_run.argtypes = [vp, i64]
+        L.fgpe_run_noalloc.argtypes = [vp, i64]
         L.fgpe_last_norms.argtypes = [vp, dp]
         L.fgpe_normc.argtypes = [vp, dp]
         L.fgpe_observables.argtypes = [vp, dp, dp, dp, dp]
@@ -188,6 +190,11 @@ class Solver:
 
     def run(self, nsteps):
         _check(lib().fgpe_run(self.h, int(nsteps)), "fgpe_run")
+
+    def run_noalloc(self, nsteps):
+        """fgpe_run with its scratch in the library's fixed device array (no
+        device allocation); calls overlapping on one GPU race."""
+        _check(lib().fgpe_run_noalloc(self.h, int(nsteps)), "fgpe_run_noalloc")
 
     def last_norms(self):
         a = np.empty(self.ms)

@@ -123,6 +123,20 @@ int fgpe_normalize(fgpe_handle* h, double* tnorm);
  * wave function is in real space and normalised as after line 495. */
 int fgpe_run(fgpe_handle* h, int64_t nsteps);
 
+/* fgpe_run without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  fgpe_run
+ * allocates its NORMC block partials on the first call; this variant runs the
+ * same kernels with those partials in a fixed 8 MiB device array that the
+ * library holds on each GPU until the process ends, and calls no hipMalloc or
+ * hipFree.  Results are bit-identical to fgpe_run.  Limit: 2^20 doubles of
+ * partials (MS per pass-A block), which every grid fgpe_create accepts fits;
+ * larger needs are refused with FGPE_EINVAL before anything runs.  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU (other
+ * handles, other streams, other threads) race and must be serialised by the
+ * caller. */
+int fgpe_run_noalloc(fgpe_handle* h, int64_t nsteps);
+
 /* NORMC of the most recent step (taken before renormalisation, as the
  * reference prints it: SRC/cgpe3d.f90:459,449,527). */
 int fgpe_last_norms(fgpe_handle* h, double* norm);
