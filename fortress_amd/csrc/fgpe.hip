@@ -83,18 +83,20 @@ template <typename V> int upload_vec(fgpe_handle* h, void** dst, const std::vect
 
 template <typename T> int build_tables(fgpe_handle* h) {
   const fgpe_params& p = h->p;
-  // each axis' twiddle tables follow the points-per-thread of the pass that
-  // transforms it: x by pass A (or the q1D kernel), y by B/D in 3D and by the
-  // k-space pass in 2D, z by the k-space pass
+  // each axis' twiddle tables follow the points-per-thread (radix plan) of
+  // the pass that transforms it: x by pass A (or the q1D kernel), y by pass B
+  // in 3D and by the k-space pass in 2D, y again by the all-component pass D
+  // in 3D (its own table, QK points per thread), z by pass C
   const int qx = h->dim == 1 ? Cfg<T>::Q1 : Cfg<T>::QX;
   const int qy = h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK;
-  const int qz = Cfg<T>::QK;
+  const int qz = Cfg<T>::QZ;
   auto kx = kgrid(h->nx, p.d[0]), ky = kgrid(h->ny, p.d[1]), kz = kgrid(h->nz, p.d[2]);
   auto xx = xgrid(h->nx, p.d[0]), yy = xgrid(h->ny, p.d[1]), zz = xgrid(h->nz, p.d[2]);
   int rc;
   if ((rc = upload_vec(h, &h->twx, stage_twiddles<T>(h->nx, qx)))) return rc;
   if ((rc = upload_vec(h, &h->twy, stage_twiddles<T>(h->ny > 1 ? h->ny : qy, qy)))) return rc;
   if ((rc = upload_vec(h, &h->twz, stage_twiddles<T>(h->nz > 1 ? h->nz : qz, qz)))) return rc;
+  if (h->dim == 3 && (rc = upload_vec(h, &h->twd, stage_twiddles<T>(h->ny, Cfg<T>::QK)))) return rc;
   const double dt = p.dt;
   const int imag = p.imag;
   // KE factors minus one, per axis (SRC/cgpe3d.f90:875-899; 2D
@@ -123,7 +125,8 @@ template <typename T> int build_tables(fgpe_handle* h) {
   if ((rc = upload_vec(h, &h->kex, kex))) return rc;
   if ((rc = upload_vec(h, &h->key, key))) return rc;
   if ((rc = upload_vec(h, &h->kel, kel))) return rc;
-  if (h->dim == 2 &&
+  // SOC factors per (ky, kx): the 2D k pass and the 3D pass D read them
+  if (h->dim >= 2 &&
       (rc = upload_vec(h, &h->soc2, soc2_table<T>(kx, ky, p.gamma[0], p.gamma[1], dt, imag, h->spin))))
     return rc;
   std::vector<T> kxt(kx.begin(), kx.end()), kyt(ky.begin(), ky.end());
@@ -340,6 +343,11 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
   h->nz = nz;
   h->npts = (long)nx * ny * nz;
   h->fp32 = fp32;
+  if (p->dim == 3) {  // default 3D schedule: eighths of each axis on two streams (fgpe_tune_schedule refines)
+    h->zchunk = nz / 8;
+    h->xchunk = nx / 8;
+    h->nstreams = 2;
+  }
   int rc = 0;
   auto bail = [&](int code) {
     fgpe_destroy(h);
@@ -353,6 +361,14 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
     return bail(fail(FGPE_EDEVICE, "hipStreamCreate failed"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return bail(fail(FGPE_EDEVICE, "hipEventCreate failed"));
+  if (p->dim == 3) {  // streams and fork/join events of the chunked 3D schedule, default priority
+    for (auto& s : h->aux)
+      if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(FGPE_EDEVICE, "hipStreamCreate failed"));
+    for (auto& e : h->evs)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(FGPE_EDEVICE, "hipEventCreate failed"));
+  }
   const size_t es = elem_size(h);
   const size_t bytes = es * (size_t)h->ms * h->npts;
   if ((rc = dalloc(h, &h->phi, bytes))) return bail(rc);
@@ -382,6 +398,13 @@ void fgpe_destroy(fgpe_handle* h) {
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (hipEvent_t e : h->evs)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t s : h->aux)
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -448,6 +471,76 @@ int fgpe_run(fgpe_handle* h, int64_t nsteps) {
   if (err)
     return fail(FGPE_ENEWTON, "Newton-Raphson could not calculate the roots within 1e-6 tolerance");
   return 0;
+}
+
+int fgpe_set_schedule(fgpe_handle* h, int32_t zchunk, int32_t xchunk, int32_t nstreams) {
+  if (!h) return fail(FGPE_EINVAL, "null handle");
+  if (zchunk < 0 || xchunk < 0 || nstreams < 1 || nstreams > 3)
+    return fail(FGPE_EINVAL, "bad schedule (%d, %d, %d)", zchunk, xchunk, nstreams);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->zchunk = h->dim == 3 ? zchunk : 0;
+  h->xchunk = h->dim == 3 ? xchunk : 0;
+  h->nstreams = h->dim == 3 ? nstreams : 1;
+  return 0;
+}
+
+int fgpe_get_schedule(fgpe_handle* h, int32_t* zchunk, int32_t* xchunk, int32_t* nstreams) {
+  if (!h || !zchunk || !xchunk || !nstreams) return fail(FGPE_EINVAL, "null argument");
+  *zchunk = h->dim == 3 ? (h->fp32 ? chunk_planes<float>(h) : chunk_planes<double>(h)) : 0;
+  *xchunk = h->dim == 3 ? (h->fp32 ? chunk_cols<float>(h) : chunk_cols<double>(h)) : 0;
+  *nstreams = h->nstreams;
+  return 0;
+}
+
+int fgpe_tune_schedule(fgpe_handle* h, int32_t trial_steps, double* ms_out, int32_t* chosen) {
+  if (!h || trial_steps < 1 || !ms_out || !chosen) return fail(FGPE_EINVAL, "bad tuning argument");
+  const int nz = h->nz, nx = h->nx;
+  const int cand[FGPE_NSCHED_CAND][3] = {{0, 0, 1},           {nz / 8, nx / 8, 2},  {nz / 4, nx / 4, 2},
+                                         {nz / 8, nx / 4, 2}, {nz / 4, nx / 8, 2},  {nz / 8, nx / 8, 3},
+                                         {nz / 16, nx / 8, 3}, {nz / 8, 0, 2}};
+  for (int i = 0; i < FGPE_NSCHED_CAND; ++i) ms_out[i] = 0.0;
+  chosen[0] = h->zchunk;
+  chosen[1] = h->xchunk;
+  chosen[2] = h->nstreams;
+  if (h->dim != 3) return 0;
+  // the trial steps advance the state: keep it, and the scalars a run
+  // updates, and put them back afterwards
+  const size_t bytes = elem_size(h) * (size_t)h->ms * h->npts;
+  void* keep = nullptr;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMalloc(&keep, bytes));
+  double sc[8 + 2];
+  auto restore = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(h->phi, keep, bytes, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->dnorm, sc, 8 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->dsig, sc + 8, 2 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+  };
+  int rc = 0;
+  if (hipMemcpyAsync(keep, h->phi, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(sc, h->dnorm, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipMemcpyAsync(sc + 8, h->dsig, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    rc = fail(FGPE_EDEVICE, "saving the state for tuning failed");
+  int best = -1;
+  for (int i = 0; i < FGPE_NSCHED_CAND && !rc; ++i) {
+    h->zchunk = cand[i][0];
+    h->xchunk = cand[i][1];
+    h->nstreams = cand[i][2];
+    if ((rc = fgpe_run(h, 1)) || (rc = fgpe_run(h, trial_steps))) break;
+    double ms = 0;
+    if ((rc = fgpe_last_run_ms(h, &ms))) break;
+    ms_out[i] = ms / trial_steps;
+    if (best < 0 || ms_out[i] < ms_out[best]) best = i;
+    if ((rc = restore())) break;
+  }
+  (void)hipFree(keep);
+  if (rc) return rc;
+  h->zchunk = cand[best][0];
+  h->xchunk = cand[best][1];
+  h->nstreams = cand[best][2];
+  return fgpe_get_schedule(h, &chosen[0], &chosen[1], &chosen[2]);
 }
 
 int fgpe_run_noalloc(fgpe_handle* h, int64_t nsteps) {
@@ -532,11 +625,17 @@ int fgpe_sync(fgpe_handle* h) {
 
 int fgpe_pass_times(fgpe_handle* h, int64_t nsteps, double* ms_out, int32_t* count_out) {
   if (!h || !ms_out || !count_out) return fail(FGPE_EINVAL, "null argument");
+  // whole-grid launches on one stream, so every line-pass launch moves the
+  // whole spinor once each way
+  const int zc = h->zchunk, xc = h->xchunk;
+  h->zchunk = h->xchunk = 0;
   h->profiling = true;
   h->ev_used = 0;
   h->ev_kind.clear();
   int rc = fgpe_run(h, nsteps);
   h->profiling = false;
+  h->zchunk = zc;
+  h->xchunk = xc;
   if (rc) return rc;
   double sum[FGPE_NPASS] = {0};
   int cnt[FGPE_NPASS] = {0};

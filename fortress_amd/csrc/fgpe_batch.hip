@@ -56,7 +56,7 @@ __global__ __launch_bounds__(TX *(N / Q)) void kg_kpass(const GInst<T>* __restri
   kpass_coords<TX * (int)sizeof(C_t<T>)>(bx, by);
   const GInst<T>& I = gi[list[bx / nb]];
   const Geom g{N, N, 1, (long)N * N};
-  kpass_block<T, 1, 2, N, Q, TX>(bx % nb, 0, I.phi, g, I.tb, I.ph, scale);
+  kpass_block<T, 1, N, Q, TX>(bx % nb, I.phi, g, I.tb, I.ph, scale);
 }
 
 template <typename T>
@@ -319,6 +319,7 @@ template <typename T> int build_batch(fgpe_batch* b) {
     }
     t.key = t.kex;  // not read by the 2D passes
     t.twz = nullptr;
+    t.twd = nullptr;
     t.cscale = I.cscale;
     ObsArgs& a = I.oa;
     a.tau0 = p.tau[0];
@@ -388,9 +389,8 @@ template <typename T, int N, int Q> int xlaunch(fgpe_batch* b, int start, int cn
 // one k-pass launch (2D: y lines, KE, SOC) over the cnt instances
 template <typename T, int N, int Q> int klaunch(fgpe_batch* b, int start, int cnt) {
   constexpr int P = N / Q;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const int nb = N / BTX;
-  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)3 * N * sizeof(R_t<T>) + (size_t)BTX * line_stride(N) * sizeof(C<T>);
+  const size_t lds = kpass_lds_bytes<T, 1, N, Q, BTX>();
   if (lds > b->lds_max) return fail(FGPE_EINVAL, "batch k pass needs %zu B of LDS", lds);
   auto kern = kg_kpass<T, N, Q, BTX>;
   allow(kern, lds);

@@ -38,8 +38,9 @@ template <> struct Cfg<float> {
   // pass A (x lines, all components): 4 points per thread keep it at <= 128
   // VGPRs (4 waves per SIMD); its SE/SP work is latency-bound, not VALU-bound
   static constexpr int QX = 4;
-  static constexpr int QY = 16;  // passes B/D (y lines, one component)
-  static constexpr int QK = 8;   // pass C (k-space lines, all components)
+  static constexpr int QY = 16;  // pass B (y lines, one component)
+  static constexpr int QZ = 16;  // 3D pass C (z lines, one component)
+  static constexpr int QK = 8;   // 2D k pass and 3D pass D (all components)
   static constexpr int Q1 = 16;  // q1D whole-step kernel
   static constexpr int TX = 16;  // x columns per strided-line block (128 B rows)
   // x columns per pass-C block: 256 threads, so three blocks share a CU and
@@ -49,6 +50,7 @@ template <> struct Cfg<float> {
 template <> struct Cfg<double> {
   static constexpr int QX = 8;
   static constexpr int QY = 8;
+  static constexpr int QZ = 8;
   static constexpr int QK = 4;
   static constexpr int Q1 = 8;
   static constexpr int TX = 8;
@@ -57,10 +59,12 @@ template <> struct Cfg<double> {
 
 // Blocks of one pass-A launch: P = NX/QX threads per x line, lines packed into
 // blocks of at least 256 threads.  Each block writes MS NORMC partials.
-template <typename T> inline long x_pass_blocks(int nx, long nlines) {
+template <typename T> inline int x_pass_lpw(int nx) {
   const int p = nx / Cfg<T>::QX;
-  const int threads = p >= 256 ? p : 256;
-  const int lpw = threads / p;
+  return (p >= 256 ? p : 256) / p;
+}
+template <typename T> inline long x_pass_blocks(int nx, long nlines) {
+  const int lpw = x_pass_lpw<T>(nx);
   return (nlines + lpw - 1) / lpw;
 }
 
@@ -188,6 +192,13 @@ struct fgpe_handle {
   bool fp32;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // 3D schedule (fgpe_set_schedule): z-planes per chunk of passes A and B,
+  // x-columns per chunk of passes C and D (0: the whole grid in one launch per
+  // pass) and streams (1-3); the extra streams and the fork/join events are
+  // created with the handle
+  int zchunk = 0, xchunk = 0, nstreams = 1;
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t evs[3] = {nullptr, nullptr, nullptr};
   int dev = 0;
   size_t lds_max = 65536;
   // state
@@ -196,6 +207,7 @@ struct fgpe_handle {
   void* stage64 = nullptr;  // FP64 staging for upload/download in FP32 mode
   // tables (T precision unless noted; twiddles and KE factors rot_of<T>)
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  void* twd = nullptr;  // 3D pass D: y axis at QK points per thread
   void *kex = nullptr, *key = nullptr, *kel = nullptr;  // KE factors (rot_of<T>)
   void* soc2 = nullptr;  // 2D SOC factor table (soc2_table)
   void *kx = nullptr, *ky = nullptr;
@@ -236,6 +248,7 @@ template <typename T> inline fgpe::Tables<T> tables(fgpe_handle* h) {
   t.twx = (const fgpe::R_t<T>*)h->twx;
   t.twy = (const fgpe::R_t<T>*)h->twy;
   t.twz = (const fgpe::R_t<T>*)h->twz;
+  t.twd = (const fgpe::R_t<T>*)h->twd;
   t.kex = (const fgpe::R_t<T>*)h->kex;
   t.key = (const fgpe::R_t<T>*)h->key;
   t.kel = (const fgpe::R_t<T>*)h->kel;
@@ -253,6 +266,24 @@ template <typename T> inline fgpe::Tables<T> tables(fgpe_handle* h) {
 }
 
 
+
+// z-planes per chunk of the 3D schedule's first phase: the handle's choice
+// when it divides NZ and its lines fill whole pass-A blocks, else the whole grid
+template <typename T> inline int chunk_planes(const fgpe_handle* h) {
+  const int zc = h->zchunk;
+  if (zc <= 0 || zc >= h->nz || h->nz % zc) return h->nz;
+  if (((long)zc * h->ny) % x_pass_lpw<T>(h->nx)) return h->nz;
+  return zc;
+}
+// x-columns per chunk of the second phase: the handle's choice when it
+// divides NX into whole y-pass blocks and pairs of k-pass blocks (the pairs
+// that share a 128 B row), else the whole grid
+template <typename T> inline int chunk_cols(const fgpe_handle* h) {
+  const int xc = h->xchunk;
+  const int q = std::max(Cfg<T>::TX, 2 * Cfg<T>::TXK);
+  if (xc <= 0 || xc >= h->nx || h->nx % xc || xc % q) return h->nx;
+  return xc;
+}
 
 // begin/end markers of one timed kernel launch (no-ops unless profiling)
 int prof_begin(fgpe_handle* h);

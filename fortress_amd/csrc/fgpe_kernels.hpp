@@ -4,11 +4,12 @@
 // float2 (FP32 mode) or double2 (FP64 mode).  One time step of the reference
 // loop (SRC/cgpe3d.f90:452-495) becomes line passes that keep whole FFT lines
 // in registers/LDS:
-//   3D: B  forward-y per component (applies the pending renormalisation)
-//       C  forward-z + KE + SOC + inverse-z, all components
-//       D  inverse-y per component
-//       A  inverse-x + 1/N + SE + SP + NORMC partials + forward-x, all comps
-//   2D: Y  forward-y + KE + SOC + inverse-y (all comps), then A
+//   3D: A  inverse-x + SE + SP + NORMC partials + forward-x, all comps
+//       B  forward-y + the kx and ky factors of KE, per component
+//       C  forward-z + the kz/GAMMAZ factor of KE + inverse-z, per component
+//          (applies the pending renormalisation)
+//       D  SOC in (kx, ky, z) space + inverse-y + 1/N, all comps
+//   2D: K  forward-y + KE + SOC + inverse-y (all comps), then A
 //   1D: one workgroup runs whole steps (k_line1d)
 // followed by a one-block finalize that turns the NORMC partials into the
 // norms and the per-component scale of the imaginary-time renormalisation.
@@ -20,15 +21,16 @@ namespace fgpe {
 enum : int { XF_INV = 1, XF_PHYS = 2, XF_FWD = 4, XF_SCALE = 8, XF_NORM = 16 };
 
 // KE (SRC/cgpe3d.f90:875-899) is a product of one factor per axis, each
-// diagonal in its wave number, so each factor is applied in the pass that has
-// that axis in k space: kx after the forward x FFT (pass A), ky after the
-// forward y FFT (pass B, 3D), and the line axis with the 3D GAMMAZ term in
-// pass C (2D: ky there).  Each factor w is stored as w - 1 (rot_of format) and
-// applied as v + v (w - 1).
+// diagonal in its wave number, so each factor is applied in a pass that has
+// that axis in k space: 2D kx after the forward x FFT (pass A) and ky in the
+// k pass; 3D kx and ky after the forward y FFT (pass B) and kz with the
+// GAMMAZ term in pass C.  Each factor w is stored as w - 1 (rot_of format)
+// and applied as v + v (w - 1).
 template <typename T> struct Tables {
   const R_t<T>* twx;  // per-stage FFT twiddles, x axis
   const R_t<T>* twy;
   const R_t<T>* twz;
+  const R_t<T>* twd;  // y axis for the 3D all-component pass D
   const R_t<T>* kex;  // exp(-i CDT kx^2) - 1                          [NX]
   const R_t<T>* key;  // exp(-i CDT ky^2) - 1 (3D)                     [NY or 1]
   const R_t<T>* kel;  // line axis: 3D exp(-i CDT (kz^2 - 2 L GAMMAZ kz)) - 1,
@@ -302,19 +304,22 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
     SFor<0, MS>::run(store);
   }
 }
+// blk0: index of the launch's first block in the whole-grid numbering, so a
+// launch over a z-chunk keeps each block's lines and NORMC partial slot
 template <typename T, int SPIN, int N, int Q, bool FULL>
 __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
-                                               double* __restrict__ partials) {
-  xpass_block<T, SPIN, N, Q, FULL>(blockIdx.x, phi, g, tb, ph, flags, partials);
+                                               double* __restrict__ partials, int blk0) {
+  xpass_block<T, SPIN, N, Q, FULL>(blk0 + (int)blockIdx.x, phi, g, tb, ph, flags, partials);
 }
 
 // ------------------------------------------------------- pass B/D (y)
-// Lines along y for one component; a block owns TX consecutive x at one z.
-// dir -1 forward (then the ky and kx factors of KE), +1 inverse (then 1/N);
-// scale applies the pending renormalisation.  3D only.
+// Lines along y for one component; a block owns TX consecutive x at one z
+// (x = x0 + TX blockIdx.x, z = z0 + blockIdx.y, so a launch may cover an x- or
+// z-chunk).  dir -1 forward (then the ky and kx factors of KE), +1 inverse
+// (then 1/N); scale applies the pending renormalisation.  3D only.
 template <typename T, int N, int Q, int TX>
 __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int dir, int scale,
-                                                       double inv_npts) {
+                                                       double inv_npts, int x0, int z0) {
   using C = C_t<T>;
   constexpr int P = N / Q;
   constexpr int LS = line_stride(N);
@@ -323,8 +328,8 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi,
   R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
   C* lds = reinterpret_cast<C*>(lds_tw + TWS);
   const int l = threadIdx.x % TX, j = threadIdx.x / TX;
-  const int x = blockIdx.x * TX + l;
-  const int z = blockIdx.y, m = blockIdx.z;
+  const int x = x0 + (int)blockIdx.x * TX + l;
+  const int z = z0 + (int)blockIdx.y, m = blockIdx.z;
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twy[i];
   const CompBuf<C> buf(phi + m * g.npts, g.npts);
   const unsigned int voff = (unsigned int)(((long)z * g.nx * g.ny + x + (long)j * g.nx) * (long)sizeof(C));
@@ -360,12 +365,133 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_ypass(C_t<T>* __restrict__ phi,
   });
 }
 
-// ---------------------------------------------- pass C (k-space lines)
-// Lines along the last axis (z in 3D, y in 2D), all components: forward FFT,
-// KE (SRC/cgpe3d.f90:875-899), SOC (904-1004), inverse FFT.  A block owns TX
-// consecutive x (and one y in 3D).
-template <typename T, int SPIN, int DIM, int N, int Q, int TX>
-__device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
+// ------------------------------------------------- 3D pass C (z lines)
+// Lines along z for one component m = blockIdx.z; a block owns TX consecutive
+// x (x = x0 + TX blockIdx.x) at one y.  Forward z FFT, the kz factor of KE
+// with the GAMMAZ term exp(-i CDT (kz^2 - 2 L GAMMAZ kz)) (SRC/cgpe3d.f90:
+// 875-899; stored minus one, applied as v + v (w - 1)), inverse z FFT, so the
+// line returns to z space: the SO coupling of SRC/cgpe3d.f90:904-1004 depends
+// on (kx, ky) only and mixes components, so it commutes with the z
+// transforms and follows in pass D, after this KE factor as in the
+// reference's order.  scale applies the pending renormalisation.
+template <typename T, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_zpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, int scale, int x0) {
+  using C = C_t<T>;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
+  C* lds = reinterpret_cast<C*>(lds_tw + TWS);
+  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int x = x0 + (int)blockIdx.x * TX + l;
+  const int y = blockIdx.y, m = blockIdx.z;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twz[i];
+  const long plane = (long)g.nx * g.ny;
+  const CompBuf<C> buf(phi + m * g.npts, g.npts);
+  const unsigned int voff = (unsigned int)(((long)y * g.nx + x + (long)j * plane) * (long)sizeof(C));
+  const unsigned int sstep = (unsigned int)(P * plane * (long)sizeof(C));
+  C v[Q];
+  const C sc = scale ? tb.cscale[m] : mk<C>((T)1, (T)0);
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    const C a = buf.ld(voff, q * sstep);
+    v[q] = scale ? cmul(a, sc) : a;
+  });
+  __syncthreads();
+  C* my = lds + l * LS;
+  line_fft<N, Q, -1>(v, my, lds_tw, j);
+  const R_t<T>* ke = tb.kel + (long)m * N;
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    v[q] = rot1(v[q], ke[j + P * q]);
+  });
+  line_fft<N, Q, +1>(v, my, lds_tw, j);
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    buf.st(v[q], voff, q * sstep);
+  });
+}
+
+// ------------------------------------------------ 3D pass D (y lines + SOC)
+// Lines along y, all components, in (kx, ky, z) space; a block owns TX
+// consecutive x at one z (x = x0 + TX bx, z = z0 + by).  The SOC factor of
+// (kx, ky) (SRC/cgpe3d.f90:904-1004, from the FP64-built table soc2) is
+// applied at every point as its change v + corr, then each component gets the
+// inverse y FFT and the 1/N of BFT.  TX * sizeof(C) = 64 B rows use the
+// paired block order of kpass_coords.
+template <typename T, int SPIN, int N, int Q, int TX> constexpr size_t dpass_lds_bytes() {
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  return (size_t)TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C_t<T>);
+}
+template <int ROW_BYTES> __device__ __forceinline__ void kpass_coords(int& bx, int& by);
+template <typename T, int SPIN, int N, int Q, int TX>
+__global__ __launch_bounds__(TX *(N / Q)) void k_dpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph,
+                                                       double inv_npts, int bx0, int z0) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int P = N / Q;
+  constexpr int LS = line_stride(N);
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
+  C* lds = reinterpret_cast<C*>(lds_tw + TWS);
+  int bx, by;
+  kpass_coords<TX * (int)sizeof(C)>(bx, by);
+  const int l = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int x = (bx0 + bx) * TX + l;
+  const int z = z0 + by;
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twd[i];
+  const unsigned int voff = (unsigned int)(((long)z * g.nx * g.ny + x + (long)j * g.nx) * (long)sizeof(C));
+  const unsigned int sstep = (unsigned int)((long)P * g.nx * (long)sizeof(C));
+  C v[MS][Q];
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    const CompBuf<C> buf(phi + m * g.npts, g.npts);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      v[m][q] = buf.ld(voff, q * sstep);
+    });
+  });
+  SFor<0, Q>::run([&](auto qi) {
+    constexpr int q = decltype(qi)::value;
+    const int ky = j + P * q;
+    C e1, dm1[MS], pt[MS], cr[MS];
+    soc_from_table<SPIN, C>(tb.soc2 + ((long)ky * g.nx + x) * (SPIN + 1), ph.imag, e1, dm1);
+    SFor<0, MS>::run([&](auto mi) { pt[decltype(mi)::value] = v[decltype(mi)::value][q]; });
+    soc_corr<SPIN, C, T>(pt, e1, dm1, cr);
+    SFor<0, MS>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      v[m][q] = cadd(v[m][q], cr[m]);
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  __syncthreads();
+  C* my = lds + l * LS;
+  // each component is stored as soon as its inverse FFT is done
+  SFor<0, MS>::run([&](auto mi) {
+    constexpr int m = decltype(mi)::value;
+    line_fft<N, Q, +1>(v[m], my, lds_tw, j);
+    const CompBuf<C> buf(phi + m * g.npts, g.npts);
+    SFor<0, Q>::run([&](auto qi) {
+      constexpr int q = decltype(qi)::value;
+      buf.st(scalec(v[m][q], inv_npts), voff, q * sstep);
+    });
+  });
+}
+
+// ------------------------------------------------- 2D k pass (y lines)
+// Lines along y, all components: forward FFT, the ky factor of KE
+// (SRC/cgpe2d.f90:812-829), SOC (SRC/cgpe2d.f90:1554-1650, factors from the
+// FP64-built table soc2), inverse FFT.  A block owns TX consecutive x.
+// LDS: stage twiddles | KE factors [MS][N] | one padded line per column.
+template <typename T, int SPIN, int N, int Q, int TX> constexpr size_t kpass_lds_bytes() {
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  return (size_t)TWS * sizeof(R_t<T>) + (size_t)(2 * SPIN + 1) * N * sizeof(R_t<T>) +
+         (size_t)TX * line_stride(N) * sizeof(C_t<T>);
+}
+template <typename T, int SPIN, int N, int Q, int TX>
+__device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
                                             const Phys& ph, int scale) {
   using C = C_t<T>;
   constexpr int MS = 2 * SPIN + 1;
@@ -378,19 +504,10 @@ __device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__
   C* lds = reinterpret_cast<C*>(lds_ke + MS * N);
   const int l = threadIdx.x % TX, j = threadIdx.x / TX;
   const int x = bx * TX + l;
-  const int o = DIM == 3 ? by : 0;  // y index in 3D
-  const long lstride = DIM == 3 ? (long)g.nx * g.ny : (long)g.nx;
-  const R_t<T>* tw = DIM == 3 ? tb.twz : tb.twy;
-  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tw[i];
+  for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twy[i];
   for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
-  const unsigned int voff = (unsigned int)(((long)o * g.nx + x + (long)j * lstride) * (long)sizeof(C));
-  const unsigned int sstep = (unsigned int)((long)P * lstride * (long)sizeof(C));
-  const T dt = (T)ph.dt;
-  // in 3D the SOC factors depend on (kx, ky) only: formed before the line is
-  // loaded, so their transcendental code does not run while it is live
-  C e1, dm1[MS];
-  if constexpr (DIM == 3) soc_factors<SPIN, C, T>(tb.kx[x], tb.ky[o], ph, dt, e1, dm1);
-  __builtin_amdgcn_sched_barrier(0);
+  const unsigned int voff = (unsigned int)((x + (long)j * g.nx) * (long)sizeof(C));
+  const unsigned int sstep = (unsigned int)((long)P * g.nx * (long)sizeof(C));
   C v[MS][Q];
   SFor<0, MS>::run([&](auto mi) {
     constexpr int m = decltype(mi)::value;
@@ -408,7 +525,8 @@ __device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;
     const int pos = j + P * q;
-    if constexpr (DIM == 2) soc_from_table<SPIN, C>(tb.soc2 + ((long)pos * g.nx + x) * (SPIN + 1), ph.imag, e1, dm1);
+    C e1, dm1[MS];
+    soc_from_table<SPIN, C>(tb.soc2 + ((long)pos * g.nx + x) * (SPIN + 1), ph.imag, e1, dm1);
     // line-axis KE factor, then the SOC change of the result
     C pt[MS], cr[MS];
     SFor<0, MS>::run([&](auto mi) {
@@ -435,7 +553,7 @@ __device__ __forceinline__ void kpass_block(int bx, int by, C_t<T>* __restrict__
     });
   });
 }
-// Block coordinates of pass C.  When a block's rows are 64 B (half of a
+// Block coordinates of the all-component passes.  When a block's rows are 64 B (half of a
 // 128 B line), the two blocks sharing each line are given workgroup ids 8
 // apart: workgroups go to the 8 XCDs round-robin (id mod 8), each XCD with its
 // own L2, so the pair runs on one XCD at the same time and the line is fetched
@@ -453,11 +571,11 @@ template <int ROW_BYTES> __device__ __forceinline__ void kpass_coords(int& bx, i
   by = l / nbx;
 }
 
-template <typename T, int SPIN, int DIM, int N, int Q, int TX>
+template <typename T, int SPIN, int N, int Q, int TX>
 __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
   int bx, by;
   kpass_coords<TX * (int)sizeof(C_t<T>)>(bx, by);
-  kpass_block<T, SPIN, DIM, N, Q, TX>(bx, by, phi, g, tb, ph, scale);
+  kpass_block<T, SPIN, N, Q, TX>(bx, phi, g, tb, ph, scale);
 }
 
 // ------------------------------------------------------------------- 1D

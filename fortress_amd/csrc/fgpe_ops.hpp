@@ -12,38 +12,51 @@ template <typename K> void allow_lds(K kernel, size_t bytes) {
 }
 
 // ------------------------------------------------------------- launches
-template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags) {
+// Pass A over the z-planes [z0, z0 + zc) on stream st.  The chunk's lines
+// fill whole blocks (chunk_planes), so each block keeps its whole-grid index
+// and NORMC partial slot.
+template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
   constexpr int Q = Cfg<T>::QX, P = N / Q;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const int threads = P >= 256 ? P : 256;
   const int lpw = threads / P;
-  const int blocks = (int)x_pass_blocks<T>(N, (long)h->ny * h->nz);
+  const long lines0 = (long)z0 * h->ny, lines = (long)zc * h->ny;
+  const int blocks = (int)x_pass_blocks<T>(N, lines);
+  const int blk0 = (int)(lines0 / lpw);
   const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
   if (flags & XF_PHYS) {
-    int rc = ensure_partials(h, (size_t)blocks * MS);
+    int rc = ensure_partials(h, (size_t)x_pass_blocks<T>(N, (long)h->ny * h->nz) * MS);
     if (rc) return rc;
   }
   // 3D: passes B and D apply the kx factor and 1/N
   auto kern = h->dim == 3 ? k_xpass<T, SPIN, N, Q, false> : k_xpass<T, SPIN, N, Q, true>;
   allow_lds(kern, lds);
   prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
-                     phys(h), flags, h->partials);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
+                     flags, h->partials, blk0);
   HIPCHK(hipGetLastError());
   prof_end(h, (flags & XF_PHYS) ? FGPE_PASS_X : FGPE_PASS_X0);
-  if ((flags & XF_PHYS) && (flags & XF_NORM)) {
-    prof_begin(h);
-    hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, blocks,
-                       finargs(h), h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
-    HIPCHK(hipGetLastError());
-    prof_end(h, FGPE_PASS_FIN);
-  }
   return 0;
 }
 
-template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
+// NORMC partials of a whole pass A -> norms and the pending renormalisation
+template <typename T, int SPIN, int N> int launch_fin(fgpe_handle* h) {
+  constexpr int MS = 2 * SPIN + 1;
+  const int blocks = (int)x_pass_blocks<T>(N, (long)h->ny * h->nz);
+  prof_begin(h);
+  hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, blocks,
+                     finargs(h), h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
+  HIPCHK(hipGetLastError());
+  prof_end(h, FGPE_PASS_FIN);
+  return 0;
+}
+
+// y lines of one component (3D pass B) over x in [x0, x0 + xw) and z in
+// [z0, z0 + zc) on stream st.
+template <typename T, int N>
+int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc) {
   constexpr int Q = Cfg<T>::QY, P = N / Q, TX = Cfg<T>::TX;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
@@ -51,24 +64,54 @@ template <typename T, int N> int launch_y(fgpe_handle* h, int dir, int scale) {
   auto kern = k_ypass<T, N, Q, TX>;
   allow_lds(kern, lds);
   prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(h->nx / TX, h->nz, h->ms), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h),
-                     tables<T>(h), dir, scale, 1.0 / (double)h->npts);
+  hipLaunchKernelGGL(kern, dim3(xw / TX, zc, h->ms), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h),
+                     dir, scale, 1.0 / (double)h->npts, x0, z0);
   HIPCHK(hipGetLastError());
   prof_end(h, FGPE_PASS_Y);
   return 0;
 }
 
-template <typename T, int SPIN, int DIM, int N> int launch_k(fgpe_handle* h, int scale) {
+// 2D k pass (y lines, KE, SOC), all of the grid.
+template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
   constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TXK;
-  constexpr int MS = 2 * SPIN + 1;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)MS * N * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
+  const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
-  auto kern = k_kpass<T, SPIN, DIM, N, Q, TX>;
+  auto kern = k_kpass<T, SPIN, N, Q, TX>;
   allow_lds(kern, lds);
   prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(h->nx / TX, DIM == 3 ? h->ny : 1), dim3(TX * P), lds, h->stream, (C<T>*)h->phi,
-                     geom(h), tables<T>(h), phys(h), scale);
+  hipLaunchKernelGGL(kern, dim3(h->nx / TX), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
+                     phys(h), scale);
+  HIPCHK(hipGetLastError());
+  prof_end(h, FGPE_PASS_K);
+  return 0;
+}
+
+// 3D pass C (z lines, KE) over x in [x0, x0 + xw) on stream st.
+template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw) {
+  constexpr int Q = Cfg<T>::QZ, P = N / Q, TX = Cfg<T>::TX;
+  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
+  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "z pass needs %zu B of LDS", lds);
+  auto kern = k_zpass<T, N, Q, TX>;
+  allow_lds(kern, lds);
+  prof_begin(h);
+  hipLaunchKernelGGL(kern, dim3(xw / TX, h->ny, h->ms), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h),
+                     scale, x0);
+  HIPCHK(hipGetLastError());
+  prof_end(h, FGPE_PASS_Z);
+  return 0;
+}
+
+// 3D pass D (SOC, inverse y, 1/N) over x in [x0, x0 + xw) on stream st.
+template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
+  constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TXK;
+  const size_t lds = dpass_lds_bytes<T, SPIN, N, Q, TX>();
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "d pass needs %zu B of LDS", lds);
+  auto kern = k_dpass<T, SPIN, N, Q, TX>;
+  allow_lds(kern, lds);
+  prof_begin(h);
+  hipLaunchKernelGGL(kern, dim3(xw / TX, h->nz), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
+                     1.0 / (double)h->npts, x0 / TX, 0);
   HIPCHK(hipGetLastError());
   prof_end(h, FGPE_PASS_K);
   return 0;
@@ -91,36 +134,69 @@ template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps
 }
 
 // runtime size dispatch
-template <typename T, int SPIN> int dispatch_x(fgpe_handle* h, int flags) {
+template <typename T, int SPIN> int dispatch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
   switch (h->nx) {
-    case 32: return launch_x<T, SPIN, 32>(h, flags);
-    case 64: return launch_x<T, SPIN, 64>(h, flags);
-    case 128: return launch_x<T, SPIN, 128>(h, flags);
-    case 256: return launch_x<T, SPIN, 256>(h, flags);
-    case 512: return launch_x<T, SPIN, 512>(h, flags);
+    case 32: return launch_x<T, SPIN, 32>(h, flags, st, z0, zc);
+    case 64: return launch_x<T, SPIN, 64>(h, flags, st, z0, zc);
+    case 128: return launch_x<T, SPIN, 128>(h, flags, st, z0, zc);
+    case 256: return launch_x<T, SPIN, 256>(h, flags, st, z0, zc);
+    case 512: return launch_x<T, SPIN, 512>(h, flags, st, z0, zc);
   }
   return fail(FGPE_EINVAL, "unsupported NX=%d", h->nx);
 }
-template <typename T> int dispatch_y(fgpe_handle* h, int dir, int scale) {
+template <typename T, int SPIN> int dispatch_x(fgpe_handle* h, int flags) {
+  return dispatch_x<T, SPIN>(h, flags, h->stream, 0, h->nz);
+}
+template <typename T, int SPIN> int dispatch_fin(fgpe_handle* h) {
+  switch (h->nx) {
+    case 32: return launch_fin<T, SPIN, 32>(h);
+    case 64: return launch_fin<T, SPIN, 64>(h);
+    case 128: return launch_fin<T, SPIN, 128>(h);
+    case 256: return launch_fin<T, SPIN, 256>(h);
+    case 512: return launch_fin<T, SPIN, 512>(h);
+  }
+  return fail(FGPE_EINVAL, "unsupported NX=%d", h->nx);
+}
+template <typename T>
+int dispatch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc) {
   switch (h->ny) {
-    case 32: return launch_y<T, 32>(h, dir, scale);
-    case 64: return launch_y<T, 64>(h, dir, scale);
-    case 128: return launch_y<T, 128>(h, dir, scale);
-    case 256: return launch_y<T, 256>(h, dir, scale);
-    case 512: return launch_y<T, 512>(h, dir, scale);
+    case 32: return launch_y<T, 32>(h, dir, scale, st, x0, xw, z0, zc);
+    case 64: return launch_y<T, 64>(h, dir, scale, st, x0, xw, z0, zc);
+    case 128: return launch_y<T, 128>(h, dir, scale, st, x0, xw, z0, zc);
+    case 256: return launch_y<T, 256>(h, dir, scale, st, x0, xw, z0, zc);
+    case 512: return launch_y<T, 512>(h, dir, scale, st, x0, xw, z0, zc);
   }
   return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
 }
-template <typename T, int SPIN, int DIM> int dispatch_k(fgpe_handle* h, int scale) {
-  const int n = DIM == 3 ? h->nz : h->ny;
-  switch (n) {
-    case 32: return launch_k<T, SPIN, DIM, 32>(h, scale);
-    case 64: return launch_k<T, SPIN, DIM, 64>(h, scale);
-    case 128: return launch_k<T, SPIN, DIM, 128>(h, scale);
-    case 256: return launch_k<T, SPIN, DIM, 256>(h, scale);
-    case 512: return launch_k<T, SPIN, DIM, 512>(h, scale);
+template <typename T, int SPIN> int dispatch_k(fgpe_handle* h, int scale) {
+  switch (h->ny) {
+    case 32: return launch_k<T, SPIN, 32>(h, scale);
+    case 64: return launch_k<T, SPIN, 64>(h, scale);
+    case 128: return launch_k<T, SPIN, 128>(h, scale);
+    case 256: return launch_k<T, SPIN, 256>(h, scale);
+    case 512: return launch_k<T, SPIN, 512>(h, scale);
   }
-  return fail(FGPE_EINVAL, "unsupported line length %d", n);
+  return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
+}
+template <typename T> int dispatch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw) {
+  switch (h->nz) {
+    case 32: return launch_z<T, 32>(h, scale, st, x0, xw);
+    case 64: return launch_z<T, 64>(h, scale, st, x0, xw);
+    case 128: return launch_z<T, 128>(h, scale, st, x0, xw);
+    case 256: return launch_z<T, 256>(h, scale, st, x0, xw);
+    case 512: return launch_z<T, 512>(h, scale, st, x0, xw);
+  }
+  return fail(FGPE_EINVAL, "unsupported NZ=%d", h->nz);
+}
+template <typename T, int SPIN> int dispatch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
+  switch (h->ny) {
+    case 32: return launch_d<T, SPIN, 32>(h, st, x0, xw);
+    case 64: return launch_d<T, SPIN, 64>(h, st, x0, xw);
+    case 128: return launch_d<T, SPIN, 128>(h, st, x0, xw);
+    case 256: return launch_d<T, SPIN, 256>(h, st, x0, xw);
+    case 512: return launch_d<T, SPIN, 512>(h, st, x0, xw);
+  }
+  return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
 }
 template <typename T, int SPIN> int dispatch_1d(fgpe_handle* h, long nsteps) {
   switch (h->nx) {
@@ -133,6 +209,74 @@ template <typename T, int SPIN> int dispatch_1d(fgpe_handle* h, long nsteps) {
     case 4096: return launch_1d<T, SPIN, 4096>(h, nsteps);
   }
   return fail(FGPE_EINVAL, "unsupported 1D NX=%d", h->nx);
+}
+
+// 3D step schedule, two phases per step, each a pipeline over chunks:
+//   phase 1, by z-chunks: A (inverse-x, SE, SP, NORMC, forward-x) then the
+//     next step's B (forward-y, kx and ky factors of KE) -- both act within
+//     z-planes;
+//   phase 2, by x-chunks: C (forward-z, kz/GAMMAZ factor of KE, inverse-z,
+//     one component per block), applying the pending renormalisation, then D
+//     (SOC in (kx, ky, z) space, inverse-y, 1/N, all components) -- both act
+//     within x-slabs.
+// The SO coupling depends on (kx, ky) only (SRC/cgpe3d.f90:904-1004), so it
+// commutes with the z transforms and is applied after the inverse z FFT,
+// still after the whole KE factor as in the reference's order: pass C then
+// moves one component at a time and D transforms once per component.
+// Chunks go round-robin to up to three streams, so the VALU-bound A overlaps
+// the memory-bound B of neighbouring chunks, C the heavier D; the phases are
+// joined on the handle's stream (C needs every z of B, A every x of D).  The
+// renormalisation (uniform with SO coupling, per component without it, where
+// no pass mixes components) commutes with the FFTs, KE and SOC, so C applies
+// it once NORMC of every chunk of A is in.  Every chunking gives
+// bit-identical results: each line is transformed by the same code and each
+// pass-A block keeps its NORMC partial slot.
+template <typename T, int SPIN> int run_steps_3d(fgpe_handle* h, long nsteps) {
+  const int zc = chunk_planes<T>(h), xc = chunk_cols<T>(h);
+  const int nst = h->profiling ? 1 : std::max(1, std::min(h->nstreams, 3));
+  hipStream_t st[3] = {h->stream, h->aux[0], h->aux[1]};
+  auto fork = [&]() -> int {
+    if (nst == 1) return 0;
+    HIPCHK(hipEventRecord(h->evs[0], h->stream));
+    for (int i = 1; i < nst; ++i) HIPCHK(hipStreamWaitEvent(st[i], h->evs[0], 0));
+    return 0;
+  };
+  auto join = [&]() -> int {
+    for (int i = 1; i < nst; ++i) {
+      HIPCHK(hipEventRecord(h->evs[i], st[i]));
+      HIPCHK(hipStreamWaitEvent(h->stream, h->evs[i], 0));
+    }
+    return 0;
+  };
+  auto phase1 = [&](int flags, bool fwd_y) -> int {
+    int rc;
+    if ((rc = fork())) return rc;
+    for (int z0 = 0, c = 0; z0 < h->nz; z0 += zc, ++c) {
+      if ((rc = dispatch_x<T, SPIN>(h, flags, st[c % nst], z0, zc))) return rc;
+      if (fwd_y && (rc = dispatch_y<T>(h, -1, 0, st[c % nst], 0, h->nx, z0, zc))) return rc;
+    }
+    return join();
+  };
+  auto phase2 = [&]() -> int {
+    int rc;
+    if ((rc = fork())) return rc;
+    for (int x0 = 0, c = 0; x0 < h->nx; x0 += xc, ++c) {
+      if ((rc = dispatch_z<T>(h, 1, st[c % nst], x0, xc))) return rc;
+      if ((rc = dispatch_d<T, SPIN>(h, st[c % nst], x0, xc))) return rc;
+    }
+    return join();
+  };
+  int rc;
+  if ((rc = phase1(XF_FWD, true))) return rc;  // the run's first forward-x and forward-y
+  for (long s = 1; s <= nsteps; ++s) {
+    if ((rc = phase2())) return rc;
+    // NORMC every step in imaginary time (it sets the renormalisation); in
+    // real time the scale is 1 and only the last step's norms are reported
+    const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((h->p.imag || s == nsteps) ? XF_NORM : 0);
+    if ((rc = phase1(flags, s < nsteps))) return rc;
+    if ((flags & XF_NORM) && (rc = dispatch_fin<T, SPIN>(h))) return rc;
+  }
+  return 0;
 }
 
 template <typename T, int SPIN> int run_steps(fgpe_handle* h, long nsteps) {
@@ -148,19 +292,18 @@ template <typename T, int SPIN> int run_steps(fgpe_handle* h, long nsteps) {
   }
   int rc;
   hipLaunchKernelGGL(k_set_scale_one<T>, dim3(1), dim3(64), 0, h->stream, (C<T>*)h->cscale, h->ms);
-  if ((rc = dispatch_x<T, SPIN>(h, XF_FWD))) return rc;
-  for (long s = 1; s <= nsteps; ++s) {
-    if (h->dim == 3) {
-      if ((rc = dispatch_y<T>(h, -1, 1))) return rc;
-      if ((rc = dispatch_k<T, SPIN, 3>(h, 0))) return rc;
-      if ((rc = dispatch_y<T>(h, +1, 0))) return rc;
-    } else {
-      if ((rc = dispatch_k<T, SPIN, 2>(h, 1))) return rc;
+  if (h->dim == 3) {
+    if ((rc = run_steps_3d<T, SPIN>(h, nsteps))) return rc;
+  } else {
+    if ((rc = dispatch_x<T, SPIN>(h, XF_FWD))) return rc;
+    for (long s = 1; s <= nsteps; ++s) {
+      if ((rc = dispatch_k<T, SPIN>(h, 1))) return rc;
+      // NORMC every step in imaginary time (it sets the renormalisation); in
+      // real time the scale is 1 and only the last step's norms are reported
+      const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((h->p.imag || s == nsteps) ? XF_NORM : 0);
+      if ((rc = dispatch_x<T, SPIN>(h, flags))) return rc;
+      if ((flags & XF_NORM) && (rc = dispatch_fin<T, SPIN>(h))) return rc;
     }
-    // NORMC every step in imaginary time (it sets the renormalisation); in
-    // real time the scale is 1 and only the last step's norms are reported
-    const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((h->p.imag || s == nsteps) ? XF_NORM : 0);
-    if ((rc = dispatch_x<T, SPIN>(h, flags))) return rc;
   }
   if (h->p.imag) {
     hipLaunchKernelGGL(k_apply_scale<T>, dim3(grid_for(h->npts * h->ms, 256)), dim3(256), 0, h->stream,

@@ -28,6 +28,9 @@ EXPORTS = (
     "fgpe_normalize",
     "fgpe_run",
     "fgpe_run_noalloc",
+    "fgpe_set_schedule",
+    "fgpe_get_schedule",
+    "fgpe_tune_schedule",
     "fgpe_last_norms",
     "fgpe_normc",
     "fgpe_observables",
@@ -93,6 +96,9 @@ def lib():
         L.fgpe_normalize.argtypes = [vp, dp]
         L.fgpe_run.argtypes = [vp, i64]
         L.fgpe_run_noalloc.argtypes = [vp, i64]
+        L.fgpe_set_schedule.argtypes = [vp, i32, i32, i32]
+        L.fgpe_get_schedule.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+        L.fgpe_tune_schedule.argtypes = [vp, i32, dp, C.POINTER(i32)]
         L.fgpe_last_norms.argtypes = [vp, dp]
         L.fgpe_normc.argtypes = [vp, dp]
         L.fgpe_observables.argtypes = [vp, dp, dp, dp, dp]
@@ -196,6 +202,26 @@ class Solver:
         device allocation); calls overlapping on one GPU race."""
         _check(lib().fgpe_run_noalloc(self.h, int(nsteps)), "fgpe_run_noalloc")
 
+    def set_schedule(self, zchunk, xchunk=0, nstreams=1):
+        """3D pencil schedule: z-planes per chunk of the x-line/forward-y
+        phase, x-columns per chunk of the z-line/inverse-y phase, streams."""
+        _check(lib().fgpe_set_schedule(self.h, int(zchunk), int(xchunk), int(nstreams)), "fgpe_set_schedule")
+
+    def schedule(self):
+        z, x, n = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().fgpe_get_schedule(self.h, C.byref(z), C.byref(x), C.byref(n)), "fgpe_get_schedule")
+        return z.value, x.value, n.value
+
+    NSCHED_CAND = 8
+
+    def tune_schedule(self, trial_steps=20):
+        """Time the candidate 3D schedules on this state (restored afterwards)
+        and keep the fastest: (ms per step of each candidate, chosen schedule)."""
+        ms = np.zeros(self.NSCHED_CAND)
+        chosen = (C.c_int32 * 3)()
+        _check(lib().fgpe_tune_schedule(self.h, int(trial_steps), _dp(ms), chosen), "fgpe_tune_schedule")
+        return ms, tuple(chosen)
+
     def last_norms(self):
         a = np.empty(self.ms)
         _check(lib().fgpe_last_norms(self.h, _dp(a)), "fgpe_last_norms")
@@ -229,12 +255,12 @@ class Solver:
     def sync(self):
         _check(lib().fgpe_sync(self.h), "fgpe_sync")
 
-    PASS_NAMES = ("x", "x0", "y", "k", "finalize", "line1d")
+    PASS_NAMES = ("x", "x0", "y", "k", "finalize", "line1d", "z")
 
     def pass_times(self, nsteps):
         """Run nsteps with per-kernel HIP events: {pass: (avg ms, launches)}."""
-        ms = np.zeros(6)
-        cnt = np.zeros(6, dtype=np.int32)
+        ms = np.zeros(len(self.PASS_NAMES))
+        cnt = np.zeros(len(self.PASS_NAMES), dtype=np.int32)
         _check(lib().fgpe_pass_times(self.h, int(nsteps), _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int32))),
                "fgpe_pass_times")
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PASS_NAMES) if cnt[i]}

@@ -123,6 +123,30 @@ int fgpe_normalize(fgpe_handle* h, double* tnorm);
  * wave function is in real space and normalised as after line 495. */
 int fgpe_run(fgpe_handle* h, int64_t nsteps);
 
+/* Pencil schedule of a 3D handle (no reference counterpart: FFTW's plans,
+ * SRC/cgpe3d.f90:843-854, choose only how each transform is split).  A step
+ * runs in two phases, each a pipeline over chunks: the x-line pass (inverse-x,
+ * SE, SP, NORMC, forward-x) and the forward-y pass by z-chunks of zchunk
+ * planes, then the z-line pass (KE) and the inverse-y pass (SOC, inverse-y)
+ * by x-chunks of xchunk columns.  0 (or a value that does not split the axis into whole
+ * blocks) means the whole grid in one launch per pass.  Chunks go round-robin
+ * to nstreams streams (1-3) of the handle.  Every schedule gives bit-identical
+ * results.  Other dimensions ignore it.  fgpe_get_schedule returns the chunk
+ * sizes in effect and the stream count. */
+int fgpe_set_schedule(fgpe_handle* h, int32_t zchunk, int32_t xchunk, int32_t nstreams);
+int fgpe_get_schedule(fgpe_handle* h, int32_t* zchunk, int32_t* xchunk, int32_t* nstreams);
+
+/* Schedule autotuner of a 3D handle: times trial_steps steps of the handle's
+ * own state under each of FGPE_NSCHED_CAND candidate schedules (whole grid;
+ * eighths/quarters of NZ and NX on two or three streams), restores the state
+ * (and the norms and Newton state a run updates) after each, and keeps the
+ * fastest.  ms_out[FGPE_NSCHED_CAND] receives milliseconds per step of each
+ * candidate, chosen[3] the (zchunk, xchunk, nstreams) in effect.  It
+ * allocates a temporary copy of the state.  A handle starts with eighths of
+ * each axis on two streams.  Other dimensions: no-op. */
+#define FGPE_NSCHED_CAND 8
+int fgpe_tune_schedule(fgpe_handle* h, int32_t trial_steps, double* ms_out, int32_t* chosen);
+
 /* fgpe_run without any device allocation.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  fgpe_run
@@ -166,19 +190,21 @@ int fgpe_sync(fgpe_handle* h);
  * only times the whole program, SRC/cgpe3d.f90:218,606-614). */
 int fgpe_last_run_ms(fgpe_handle* h, double* ms);
 
-/* Per-kernel device times of a timed run: advances nsteps steps exactly like
- * fgpe_run while recording HIP events around every kernel launch on the
- * handle's stream, and returns in ms_out[FGPE_NPASS] the average duration in
- * milliseconds of each pass kind (0 where a kind did not run) and in
+/* Per-kernel device times of a timed run: advances nsteps steps like
+ * fgpe_run, with whole-grid launches on the handle's stream (one launch per
+ * pass and step, whatever the schedule), while recording HIP events around
+ * every kernel launch, and returns in ms_out[FGPE_NPASS] the average duration
+ * in milliseconds of each pass kind (0 where a kind did not run) and in
  * count_out[FGPE_NPASS] how many launches it averaged.  No reference
  * counterpart; it backs the per-pass bandwidth report. */
 #define FGPE_PASS_X 0     /* inverse-x + SE + SP + NORMC (+ forward-x) */
 #define FGPE_PASS_X0 1    /* forward-x only (first step of a run) */
-#define FGPE_PASS_Y 2     /* y lines, one component (3D) */
-#define FGPE_PASS_K 3     /* k-space lines with KE + SOC */
+#define FGPE_PASS_Y 2     /* 3D y lines, one component: forward-y, kx/ky KE */
+#define FGPE_PASS_K 3     /* 2D k-space lines (KE + SOC); 3D y lines with SOC + inverse-y */
 #define FGPE_PASS_FIN 4   /* NORMC finalisation / renormalisation scale */
 #define FGPE_PASS_1D 5    /* q1D whole-step kernel */
-#define FGPE_NPASS 6
+#define FGPE_PASS_Z 6     /* 3D z lines, one component: forward-z, KE, inverse-z */
+#define FGPE_NPASS 7
 int fgpe_pass_times(fgpe_handle* h, int64_t nsteps, double* ms_out, int32_t* count_out);
 
 /* Message of the last failing call on this thread. */
