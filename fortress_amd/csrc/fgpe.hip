@@ -378,6 +378,7 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
   if ((rc = dalloc(h, &h->cscale, 5 * sizeof(double2)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->cscale64, 8 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->dnorm, 8 * sizeof(double)))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->obs_sums, 32 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->dsig, 2 * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->derr, sizeof(int)))) return bail(rc);
   const double sig0[2] = {20.0, 20.0};  // SRC/cgpe3d.f90:443-444

@@ -218,6 +218,7 @@ struct fgpe_handle {
   void* cscale = nullptr;        // [ms] complex T
   double* cscale64 = nullptr;    // [ms]
   double* dnorm = nullptr;       // [8]: norms, TNORM
+  double* obs_sums = nullptr;    // [32]: the observables' column integrals
   double* dsig = nullptr;        // Newton state
   int* derr = nullptr;
   double* partials = nullptr;    // reduction scratch

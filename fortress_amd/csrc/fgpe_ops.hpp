@@ -356,10 +356,60 @@ template <typename T> int normalize_t(fgpe_handle* h, double* tnorm) {
   return 0;
 }
 
+// Tiles of the observables line pass along `axis` (lines per tile from the
+// axis' line length), and the blocks that loop over them.
+template <typename T, int SPIN> long obs_axis_tiles(const fgpe_handle* h, int axis) {
+  const int n = axis == 0 ? h->nx : axis == 1 ? h->ny : h->nz;
+  const int L = n == 32    ? obs_lines<T, SPIN, 32>()
+                : n == 64  ? obs_lines<T, SPIN, 64>()
+                : n == 128 ? obs_lines<T, SPIN, 128>()
+                : n == 256 ? obs_lines<T, SPIN, 256>()
+                           : obs_lines<T, SPIN, 512>();
+  const long nbx = (h->nx + L - 1) / L;
+  return axis == 0 ? ((long)h->ny * h->nz + L - 1) / L : axis == 1 ? nbx * h->nz : nbx * h->ny;
+}
+template <typename T, int SPIN> long obs_axis_blocks(const fgpe_handle* h, int axis) {
+  return std::min(obs_axis_tiles<T, SPIN>(h, axis), OBS_MAX_BLOCKS);
+}
+
+// One axis of the 2D/3D observables (k_obs_line) for lines of N points;
+// returns the number of blocks launched, whose sums start at partials + off
+// (the caller has sized the partials for every axis).
+template <typename T, int SPIN, int DIM, int AXIS, int N>
+int launch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk) {
+  const size_t lds = obs_line_lds<T, SPIN, N>();
+  if (lds > h->lds_max) return fail(FGPE_EINVAL, "observables need %zu B of LDS", lds);
+  const long blocks = obs_axis_blocks<T, SPIN>(h, AXIS);
+  auto kern = k_obs_line<T, SPIN, DIM, AXIS, N>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(OBS_TPB), lds, h->stream, (const C<T>*)h->phi, geom(h), a,
+                     obs_axis_tiles<T, SPIN>(h, AXIS), h->partials + off);
+  HIPCHK(hipGetLastError());
+  *nblk = (int)blocks;
+  return 0;
+}
+template <typename T, int SPIN, int DIM, int AXIS>
+int dispatch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk) {
+  const int n = AXIS == 0 ? h->nx : AXIS == 1 ? h->ny : h->nz;
+  switch (n) {
+    case 32: return launch_obs_axis<T, SPIN, DIM, AXIS, 32>(h, a, off, nblk);
+    case 64: return launch_obs_axis<T, SPIN, DIM, AXIS, 64>(h, a, off, nblk);
+    case 128: return launch_obs_axis<T, SPIN, DIM, AXIS, 128>(h, a, off, nblk);
+    case 256: return launch_obs_axis<T, SPIN, DIM, AXIS, 256>(h, a, off, nblk);
+    case 512: return launch_obs_axis<T, SPIN, DIM, AXIS, 512>(h, a, off, nblk);
+  }
+  return fail(FGPE_EINVAL, "unsupported line length %d", n);
+}
+
 template <typename T, int SPIN> int observables_t(fgpe_handle* h, double* en, double* mu, double* mz, double* rms) {
-  const int blocks = std::min(grid_for(h->npts, 256), 4096);
+  int blocks = std::min(grid_for(h->npts, 256), 4096);
   const int nc = 3 * h->ms + 2;
-  int rc = ensure_partials(h, (size_t)blocks * nc);
+  long need = blocks;
+  if (h->dim >= 2) {
+    need = 0;
+    for (int ax = 0; ax < h->dim; ++ax) need += obs_axis_blocks<T, SPIN>(h, ax);
+  }
+  int rc = ensure_partials(h, (size_t)need * nc);
   if (rc) return rc;
   ObsArgs a;
   a.tau0 = h->p.tau[0];
@@ -382,21 +432,30 @@ template <typename T, int SPIN> int observables_t(fgpe_handle* h, double* en, do
   a.vzd = h->vzd;
   const C<T>* ph = (const C<T>*)h->phi;
   Geom g = geom(h);
-#define OBS(D) hipLaunchKernelGGL((k_observables<T, SPIN, D>), dim3(blocks), dim3(256), 0, h->stream, ph, g, a, h->partials)
-  if (h->dim == 1)
-    OBS(1);
-  else if (h->dim == 2)
-    OBS(2);
-  else
-    OBS(3);
-#undef OBS
+  if (h->dim == 1) {
+    hipLaunchKernelGGL((k_observables<T, SPIN, 1>), dim3(blocks), dim3(256), 0, h->stream, ph, g, a, h->partials);
+    HIPCHK(hipGetLastError());
+  } else {
+    // one read-only line pass per axis; the x pass adds the pointwise terms
+    int b0 = 0, b1 = 0, b2 = 0;
+    if (h->dim == 2) {
+      if ((rc = dispatch_obs_axis<T, SPIN, 2, 0>(h, a, 0, &b0))) return rc;
+      if ((rc = dispatch_obs_axis<T, SPIN, 2, 1>(h, a, (size_t)b0 * nc, &b1))) return rc;
+    } else {
+      if ((rc = dispatch_obs_axis<T, SPIN, 3, 0>(h, a, 0, &b0))) return rc;
+      if ((rc = dispatch_obs_axis<T, SPIN, 3, 1>(h, a, (size_t)b0 * nc, &b1))) return rc;
+      if ((rc = dispatch_obs_axis<T, SPIN, 3, 2>(h, a, (size_t)(b0 + b1) * nc, &b2))) return rc;
+    }
+    blocks = b0 + b1 + b2;
+  }
+  // the block sums reduced on the device in a fixed order; NC values come back
+  constexpr int NC = 3 * (2 * SPIN + 1) + 2;
+  hipLaunchKernelGGL((k_sum_rows<NC>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, (long)blocks,
+                     h->obs_sums);
   HIPCHK(hipGetLastError());
-  std::vector<double> part((size_t)blocks * nc);
-  HIPCHK(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  double s[NC];
+  HIPCHK(hipMemcpyAsync(s, h->obs_sums, sizeof s, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  std::vector<double> s(nc, 0.0);
-  for (int b = 0; b < blocks; ++b)
-    for (int c = 0; c < nc; ++c) s[c] += part[(size_t)b * nc + c];
   const int ms = h->ms;
   double sumn = 0;
   for (int m = 0; m < ms; ++m) sumn += s[2 * ms + 2 + m];

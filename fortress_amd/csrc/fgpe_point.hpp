@@ -248,6 +248,309 @@ __global__ __launch_bounds__(256) void k_observables(const C_t<T>* __restrict__ 
   block_sum_store<NC>(acc, red, partials);
 }
 
+// ------------------------------------------- observables as line passes
+// ENERGY's integrand splits into a pointwise part and one part per axis d
+// that involves only the derivatives along d (SRC/cgpe3d.f90:1381-1503,
+// SRC/cgpe2d.f90:1145-1267, SRC/cgpe1d.f90:1150-1233):
+//   kinetic  sum_m |d_d phi_m|^2           (EN and MU_m columns)
+//   SO       Re(-i gamma_d phi^H (2 S_d) d_d phi) = gamma_d Im(phi^H u)
+//            with u = 2 S_d d_d phi, and per component gamma_d Im(phi_m^* u_m)
+//            for MU_m (standard S_x, S_y; S_z diagonal, 3D only).
+// So each axis is one read-only pass over lines along it: the lines of all
+// components sit in LDS, DIFF (SRC/diff.f90:4-33) reads its stencil from
+// there, and the x pass also adds the pointwise terms, RAD and NORMC.
+
+// u = 2 S_d v for axis d (0 x, 1 y, 2 z)
+template <int SPIN, int AXIS> __device__ __forceinline__ void two_s(const Z* v, Z* u) {
+  const Z I(0.0, 1.0);
+  if constexpr (SPIN == 1) {
+    const double r2 = 1.41421356237309504880;
+    if constexpr (AXIS == 0) {
+      u[0] = r2 * v[1];
+      u[1] = r2 * (v[0] + v[2]);
+      u[2] = r2 * v[1];
+    } else if constexpr (AXIS == 1) {
+      u[0] = (-r2) * (I * v[1]);
+      u[1] = r2 * (I * (v[0] - v[2]));
+      u[2] = r2 * (I * v[1]);
+    } else {
+      u[0] = 2.0 * v[0];
+      u[1] = Z();
+      u[2] = (-2.0) * v[2];
+    }
+  } else {
+    const double r6 = 2.44948974278317809820;
+    if constexpr (AXIS == 0) {
+      u[0] = 2.0 * v[1];
+      u[1] = 2.0 * v[0] + r6 * v[2];
+      u[2] = r6 * (v[1] + v[3]);
+      u[3] = r6 * v[2] + 2.0 * v[4];
+      u[4] = 2.0 * v[3];
+    } else if constexpr (AXIS == 1) {
+      u[0] = (-2.0) * (I * v[1]);
+      u[1] = I * (2.0 * v[0] - r6 * v[2]);
+      u[2] = r6 * (I * (v[1] - v[3]));
+      u[3] = I * (r6 * v[2] - 2.0 * v[4]);
+      u[4] = 2.0 * (I * v[3]);
+    } else {
+      u[0] = 4.0 * v[0];
+      u[1] = 2.0 * v[1];
+      u[2] = Z();
+      u[3] = (-2.0) * v[3];
+      u[4] = (-4.0) * v[4];
+    }
+  }
+}
+
+// Pointwise columns at one point (no derivatives): EN's V rho + interaction
+// energy, MZ, the MU_m numerators' potential/interaction parts, RAD's
+// r^2 |phi_m|^2 and NORMC's |phi_m|^2, as in obs_point.
+template <int SPIN> __device__ __forceinline__ void obs_pointwise(const Z* ph, double v, double r2, const ObsArgs& a,
+                                                                  double* c) {
+  constexpr int MS = 2 * SPIN + 1;
+  const double t0 = a.tau0, t1 = a.tau1, t2 = a.tau2;
+  const double s2 = 1.41421356237309504880, s15 = 1.22474487139158904910;
+  const Z IM(0.0, 1.0);
+  Z cph[MS];
+  double am[MS], rho = 0;
+#pragma unroll
+  for (int m = 0; m < MS; ++m) {
+    cph[m] = cj(ph[m]);
+    am[m] = ab2(ph[m]);
+    rho += am[m];
+  }
+  if constexpr (SPIN == 1) {
+    const double s5 = 0.70710678118654752440;
+    Z sxp[3] = {s5 * ph[1], s5 * (ph[0] + ph[2]), s5 * ph[1]};
+    Z syp[3] = {(IM / s2) * ph[1], (IM / s2) * (-ph[0] + ph[2]), (IM / s2) * (-ph[1])};
+    Z ax, ay;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      ax = ax + cph[m] * sxp[m];
+      ay = ay + cph[m] * syp[m];
+    }
+    const double fx = ax.r, fy = ay.r, fz = am[0] - am[2];
+    c[0] += v * rho + t0 * rho * rho / 2.0 + t1 * (fx * fx + fy * fy + fz * fz) / 2.0;
+    c[1] += am[0] - am[2];
+    c[2] += (Z(v * am[0] + t0 * (am[0] + am[1] + am[2]) * am[0] + t1 * (am[0] + am[1] - am[2]) * am[0]) +
+             t1 * cph[2] * ph[1] * ph[1] * cph[0]).r;
+    c[3] += (Z(v * am[1] + t0 * (am[0] + am[1] + am[2]) * am[1] + t1 * (am[0] + am[2]) * am[1]) +
+             2.0 * t1 * ph[0] * ph[2] * cph[1] * cph[1]).r;
+    c[4] += (Z(v * am[2] + t0 * (am[0] + am[1] + am[2]) * am[2] + t1 * (am[1] + am[2] - am[0]) * am[2]) +
+             t1 * cph[0] * ph[1] * ph[1] * cph[2]).r;
+  } else {
+    const double r = s15;
+    Z sxp[5] = {ph[1], ph[0] + r * ph[2], r * (ph[1] + ph[3]), r * ph[2] + ph[4], ph[3]};
+    Z syp[5] = {IM * (-ph[1]), IM * (ph[0] - r * ph[2]), IM * (r * ph[1] - r * ph[3]), IM * (r * ph[2] - ph[4]),
+                IM * ph[3]};
+    Z ax, ay;
+    double fz = 0;
+    const double sz[5] = {2, 1, 0, -1, -2};
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      ax = ax + cph[m] * sxp[m];
+      ay = ay + cph[m] * syp[m];
+      fz += sz[m] * am[m];
+    }
+    const double fx = ax.r, fy = ay.r;
+    Z fm = Z(fx, -fy);
+    Z cc[6];
+    cc[0] = t1 * fm - 0.4 * t2 * ph[3] * cj(ph[4]);
+    cc[1] = 0.2 * t2 * ph[2] * cj(ph[4]);
+    cc[2] = s15 * t1 * fm - 0.2 * t2 * ph[2] * cj(ph[3]);
+    cc[3] = s15 * t1 * fm - 0.2 * t2 * ph[1] * cj(ph[2]);
+    cc[4] = 0.2 * t2 * ph[0] * cj(ph[2]);
+    cc[5] = t1 * fm - 0.4 * t2 * ph[0] * cj(ph[1]);
+    Z th = (2.0 * ph[0] * ph[4] - 2.0 * ph[1] * ph[3] + ph[2] * ph[2]) / 2.23606797749978969641;
+    c[0] += v * rho + t0 * rho * rho / 2.0 + t1 * (fx * fx + fy * fy + fz * fz) / 2.0 + t2 * ab2(th) / 2.0;
+    c[1] += 2.0 * am[0] + am[1] - am[3] - 2.0 * am[4];
+    c[2] += (Z((v + t0 * rho + 2.0 * t1 * fz + 0.4 * t2 * am[4]) * am[0]) + (cc[0] * ph[1] + cc[1] * ph[2]) * cph[0]).r;
+    c[3] += (Z((v + t0 * rho + t1 * fz + 0.4 * t2 * am[3]) * am[1]) + (cj(cc[0]) * ph[0] + cc[2] * ph[2]) * cph[1]).r;
+    c[4] += (Z((v + t0 * rho + 0.2 * t2 * am[2]) * am[2]) +
+             (cj(cc[1]) * ph[0] + cj(cc[2]) * ph[1] + cc[3] * ph[3] + cc[4] * ph[4]) * cph[2]).r;
+    c[5] += (Z((v + t0 * rho - t1 * fz + 0.4 * t2 * am[1]) * am[3]) + (cj(cc[3]) * ph[2] + cc[5] * ph[4]) * cph[3]).r;
+    c[6] += (Z((v + t0 * rho - 2.0 * t1 * fz + 0.4 * t2 * am[0]) * am[4]) +
+             (cj(cc[4]) * ph[2] + cj(cc[5]) * ph[3]) * cph[4]).r;
+  }
+#pragma unroll
+  for (int m = 0; m < MS; ++m) {
+    c[MS + 2 + m] += r2 * am[m];
+    c[2 * MS + 2 + m] += am[m];
+  }
+}
+
+// Lines per tile of an observables line pass: all MS components of L lines
+// of N points (padded to N + 2) in at most 48 KB of LDS (three blocks per
+// CU; 256^3 spin-2 FP32: 4 lines, 32 B rows for the y and z passes), L a
+// power of two, at most 64 (at least 1).
+template <typename T, int SPIN, int N> __host__ __device__ constexpr int obs_lines() {
+  constexpr long per = (long)(2 * SPIN + 1) * (N + 2) * sizeof(C_t<T>);
+  int l = 1;
+  while (l < 64 && (long)(2 * l) * per <= 49152) l *= 2;
+  return l;
+}
+// blocks of an observables line pass: each loops over tiles, so the partial
+// rows (one per block) stay few
+constexpr long OBS_MAX_BLOCKS = 1024;
+template <typename T, int SPIN, int N> __host__ __device__ constexpr size_t obs_line_lds() {
+  return (size_t)(2 * SPIN + 1) * obs_lines<T, SPIN, N>() * (N + 2) * sizeof(C_t<T>);
+}
+// threads per observables block
+constexpr int OBS_TPB = 256;
+
+// Sum of nparts rows of NC partials into out[NC] in a fixed order (one block
+// of 1024 threads: thread t takes rows t, t + 1024, ...; then a fixed tree),
+// so the observables are bit-reproducible and only NC values leave the device.
+template <int NC>
+__global__ __launch_bounds__(1024) void k_sum_rows(const double* __restrict__ partials, long nparts,
+                                                   double* __restrict__ out) {
+  __shared__ double red[16 * NC];
+  double a[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) a[c] = 0.0;
+  for (long r = threadIdx.x; r < nparts; r += 1024)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] += partials[r * NC + c];
+  wave_sum<NC>(a);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) red[wid * NC + c] = a[c];
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    double s = 0;
+    for (int w = 0; w < 16; ++w) s += red[w * NC + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// One axis of the observables (2D/3D): tiles of L lines along AXIS (x: L
+// consecutive lines of the (y, z) grid; y, z: L consecutive x columns at one
+// z or y), all components loaded once into LDS; each block takes tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the ntiles, each thread adds
+// the Simpson-weighted columns of its points to FP64 sums, and a block stores
+// its 3 MS + 2 sums in partials[blockIdx.x].  Lines past the grid (a last
+// partial tile) are skipped.
+template <typename T, int SPIN, int DIM, int AXIS, int N>
+__global__ __launch_bounds__(OBS_TPB) void k_obs_line(const C_t<T>* __restrict__ phi, Geom g, ObsArgs a, long ntiles,
+                                                      double* __restrict__ partials) {
+  using C = C_t<T>;
+  constexpr int MS = 2 * SPIN + 1;
+  constexpr int NC = 3 * MS + 2;
+  constexpr int L = obs_lines<T, SPIN, N>();
+  constexpr int NS = N + 2;   // padded line stride
+  constexpr int PTS = L * N;  // points of a tile
+  static_assert(PTS % OBS_TPB == 0, "observables block must tile its points");
+  constexpr int PER = PTS / OBS_TPB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);  // [MS][L][NS]
+  static_assert(16 * NC * sizeof(double) <= (size_t)MS * L * NS * sizeof(C), "reduction must fit the tile");
+  const long nlines = (long)g.ny * g.nz;
+  const int nbx = (g.nx + L - 1) / L;
+  const double d = AXIS == 0 ? a.dx : AXIS == 1 ? a.dy : a.dz;
+  const double gam = AXIS == 0 ? a.gx : AXIS == 1 ? a.gy : a.gz;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  for (long b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    // line l of the tile: its (y, z) (x lines) or x (y, z lines) and validity
+    long line0 = 0;
+    int ox = 0, oy = 0, oz = 0;
+    if constexpr (AXIS == 0) {
+      line0 = b * L;
+    } else if constexpr (AXIS == 1) {
+      ox = (int)(b % nbx) * L;
+      oz = (int)(b / nbx);
+    } else {
+      ox = (int)(b % nbx) * L;
+      oy = (int)(b / nbx);
+    }
+    auto valid = [&](int l) { return AXIS == 0 ? line0 + l < nlines : ox + l < g.nx; };
+    auto gidx = [&](int l, int i) -> long {
+      if constexpr (AXIS == 0) return (line0 + l) * g.nx + i;
+      else if constexpr (AXIS == 1) return ((long)oz * g.ny + i) * g.nx + ox + l;
+      else return ((long)i * g.ny + oy) * g.nx + ox + l;
+    };
+    __syncthreads();  // the previous tile's readers are done with the LDS
+    // load: x lines contiguous (i fastest); y/z lines with the L columns
+    // fastest; every load of a component is issued before its LDS stores
+#pragma unroll
+    for (int m = 0; m < MS; ++m) {
+      const C* pm = phi + m * g.npts;
+      C tmp[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * OBS_TPB;
+        const int l = AXIS == 0 ? e / N : e % L, i = AXIS == 0 ? e % N : e / L;
+        tmp[k] = valid(l) ? pm[gidx(l, i)] : mk<C>((T)0, (T)0);
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * OBS_TPB;
+        const int l = AXIS == 0 ? e / N : e % L, i = AXIS == 0 ? e % N : e / L;
+        lds[(m * L + l) * NS + i] = tmp[k];
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < PTS; e += OBS_TPB) {
+      const int l = e / N, i = e % N;
+      if (!valid(l)) continue;
+      int x, y, z;
+      if constexpr (AXIS == 0) {
+        x = i;
+        y = (int)((line0 + l) % g.ny);
+        z = (int)((line0 + l) / g.ny);
+      } else if constexpr (AXIS == 1) {
+        x = ox + l;
+        y = i;
+        z = oz;
+      } else {
+        x = ox + l;
+        y = oy;
+        z = i;
+      }
+      Z ph[MS], dp[MS], u[MS];
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const C* line = lds + (m * L + l) * NS;
+        const C c0 = line[i];
+        ph[m] = Z((double)c0.x, (double)c0.y);
+        dp[m] = diff_at(
+            [&](int t) {
+              const C v = line[t];
+              return Z((double)v.x, (double)v.y);
+            },
+            i, N, d);
+      }
+      double c[NC];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) c[k] = 0.0;
+      two_s<SPIN, AXIS>(dp, u);
+      double so = 0.0;
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const double k2 = ab2(dp[m]);
+        const double sm = gam * (ph[m].r * u[m].i - ph[m].i * u[m].r);  // gamma Im(phi_m^* u_m)
+        so += sm;
+        c[0] += k2;
+        c[2 + m] += k2 + sm;
+      }
+      c[0] += so;
+      if constexpr (AXIS == 0) {
+        const double v = a.vxd[x] + (DIM >= 2 ? a.vyd[y] : 0.0) + (DIM >= 3 ? a.vzd[z] : 0.0);
+        const double r2 = a.x2[x] + (DIM >= 2 ? a.y2[y] : 0.0) + (DIM >= 3 ? a.z2[z] : 0.0);
+        obs_pointwise<SPIN>(ph, v, r2, a, c);
+      }
+      const double w = a.wx[x] * (DIM >= 2 ? a.wy[y] : 1.0) * (DIM >= 3 ? a.wz[z] : 1.0);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) acc[k] += w * c[k];
+    }
+  }
+  // the wave sums reuse the tile's LDS once every thread is done with it
+  // (block_sum_store synchronises before writing)
+  block_sum_store<NC>(acc, reinterpret_cast<double*>(smem_raw), partials, (long)blockIdx.x);
+}
+
 // NORMC partials over real-space points (separable Simpson weights)
 template <typename T, int MS>
 __global__ __launch_bounds__(256) void k_norm_reduce(const C_t<T>* __restrict__ phi, Geom g, const double* wx,
