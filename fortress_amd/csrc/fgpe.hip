@@ -87,16 +87,16 @@ template <typename T> int build_tables(fgpe_handle* h) {
   // the pass that transforms it: x by pass A (or the q1D kernel), y by pass B
   // in 3D and by the k-space pass in 2D, y again by the all-component pass D
   // in 3D (its own table, QK points per thread), z by pass C
-  const int qx = h->dim == 1 ? Cfg<T>::Q1 : Cfg<T>::QX;
-  const int qy = h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK;
-  const int qz = Cfg<T>::QZ;
+  const int qx = q_for(h->nx, h->dim == 1 ? Cfg<T>::Q1 : Cfg<T>::QX);
+  const int qy = q_for(h->ny, h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK);
+  const int qz = q_for(h->nz, Cfg<T>::QZ);
   auto kx = kgrid(h->nx, p.d[0]), ky = kgrid(h->ny, p.d[1]), kz = kgrid(h->nz, p.d[2]);
   auto xx = xgrid(h->nx, p.d[0]), yy = xgrid(h->ny, p.d[1]), zz = xgrid(h->nz, p.d[2]);
   int rc;
   if ((rc = upload_vec(h, &h->twx, stage_twiddles<T>(h->nx, qx)))) return rc;
   if ((rc = upload_vec(h, &h->twy, stage_twiddles<T>(h->ny > 1 ? h->ny : qy, qy)))) return rc;
   if ((rc = upload_vec(h, &h->twz, stage_twiddles<T>(h->nz > 1 ? h->nz : qz, qz)))) return rc;
-  if (h->dim == 3 && (rc = upload_vec(h, &h->twd, stage_twiddles<T>(h->ny, Cfg<T>::QK)))) return rc;
+  if (h->dim == 3 && (rc = upload_vec(h, &h->twd, stage_twiddles<T>(h->ny, q_for(h->ny, Cfg<T>::QK))))) return rc;
   const double dt = p.dt;
   const int imag = p.imag;
   // KE factors minus one, per axis (SRC/cgpe3d.f90:875-899; 2D
@@ -234,8 +234,10 @@ int grid_for(long n, int threads) {
 }
 
 static bool supported_line(int n, int dim1) {
-  if (dim1) return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
-  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+#define FGPE_CASE(s) || n == s
+  if (dim1) return false FGPE_1D_SIZES(FGPE_CASE);
+  return false FGPE_LINE_SIZES(FGPE_CASE);
+#undef FGPE_CASE
 }
 
 static size_t elem_size(const fgpe_handle* h) { return h->fp32 ? sizeof(float2) : sizeof(double2); }
@@ -316,10 +318,11 @@ int fgpe_create(const fgpe_params* p, fgpe_handle** out) {
   const int TX = fp32 ? Cfg<float>::TX : Cfg<double>::TX;
   const int nx = p->n[0], ny = p->dim >= 2 ? p->n[1] : 1, nz = p->dim >= 3 ? p->n[2] : 1;
   if (p->dim == 1) {
-    if (!supported_line(nx, 1)) return fail(FGPE_EINVAL, "NX=%d not supported in 1D (64..4096, power of 2)", nx);
+    if (!supported_line(nx, 1))
+      return fail(FGPE_EINVAL, "NX=%d not supported in 1D (2^a or 3*2^a, 64..4096)", nx);
   } else {
     if (!supported_line(nx, 0) || !supported_line(ny, 0) || (p->dim == 3 && !supported_line(nz, 0)))
-      return fail(FGPE_EINVAL, "grid %dx%dx%d not supported (each axis 32..512, power of 2)", nx, ny, nz);
+      return fail(FGPE_EINVAL, "grid %dx%dx%d not supported (each axis 2^a or 3*2^a, 32..512)", nx, ny, nz);
     if (nx % TX) return fail(FGPE_EINVAL, "NX must be a multiple of %d", TX);
   }
   int ndev = 0;

@@ -57,10 +57,21 @@ template <> struct Cfg<double> {
   static constexpr int TXK = 8;
 };
 
+// Line lengths with compiled passes: 2D/3D axes, and the q1D line.  A
+// length is 2^a or 3 * 2^a; its FFT is planned from radix 16/12/8/6/4/3/2
+// stages (pick_radix).
+#define FGPE_LINE_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#define FGPE_1D_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
+
+// Points per thread of a pass over lines of n points: the pass's default
+// (Cfg) for powers of two; for n = 3 * 2^a a multiple of 6, so the radix-3
+// factor has a stage (12 where the default is 16, else 6).
+__host__ __device__ constexpr int q_for(int n, int qdef) { return (n & (n - 1)) == 0 ? qdef : (qdef >= 12 ? 12 : 6); }
+
 // Blocks of one pass-A launch: P = NX/QX threads per x line, lines packed into
 // blocks of at least 256 threads.  Each block writes MS NORMC partials.
 template <typename T> inline int x_pass_lpw(int nx) {
-  const int p = nx / Cfg<T>::QX;
+  const int p = nx / q_for(nx, Cfg<T>::QX);
   return (p >= 256 ? p : 256) / p;
 }
 template <typename T> inline long x_pass_blocks(int nx, long nlines) {
@@ -284,6 +295,24 @@ template <typename T> inline int chunk_cols(const fgpe_handle* h) {
   const int q = std::max(Cfg<T>::TX, 2 * Cfg<T>::TXK);
   if (xc <= 0 || xc >= h->nx || h->nx % xc || xc % q) return h->nx;
   return xc;
+}
+
+// Tiles of the observables line pass along `axis` (lines per tile from the
+// axis' line length), and the blocks that loop over them.
+template <typename T, int SPIN> long obs_axis_tiles(const fgpe_handle* h, int axis) {
+  const int n = axis == 0 ? h->nx : axis == 1 ? h->ny : h->nz;
+  int L = 1;
+  switch (n) {
+#define FGPE_CASE(s) \
+  case s: L = obs_lines<T, SPIN, s>(); break;
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
+  }
+  const long nbx = (h->nx + L - 1) / L;
+  return axis == 0 ? ((long)h->ny * h->nz + L - 1) / L : axis == 1 ? nbx * h->nz : nbx * h->ny;
+}
+template <typename T, int SPIN> long obs_axis_blocks(const fgpe_handle* h, int axis) {
+  return std::min(obs_axis_tiles<T, SPIN>(h, axis), OBS_MAX_BLOCKS);
 }
 
 // begin/end markers of one timed kernel launch (no-ops unless profiling)

@@ -77,7 +77,9 @@ __device__ __forceinline__ int newton_dev(double* x, const double* n, double mag
     x[1] += p1;
     if (fmax(fabs(p0), fabs(p1)) <= 1.0e-6) break;
   }
-  return fmax(fabs(p0), fabs(p1)) > 1.0e-6;
+  // a step that is not <= TOL_NR fails, a NaN one included (the reference's
+  // MAXVAL(ABS(P)).GT.TOL_NR lets NaN through and the state turns NaN)
+  return !(fabs(p0) <= 1.0e-6 && fabs(p1) <= 1.0e-6);
 }
 
 // norms n[ms] -> per-component scale c[ms] applied after the step:

@@ -1,146 +1,32 @@
-// fgpe_ops.hpp -- templated launch code of libfgpe: pass launches with their
-// LDS budgets, runtime size dispatch, the per-run pass schedule and the
-// event-time reductions.  Instantiated per precision and spin in
-// fgpe_ops_*.hip so the kernels compile in parallel.
+// fgpe_ops.hpp -- templated host code of libfgpe: runtime size dispatch to
+// the pass launches (fgpe_launch.hpp, one translation unit per line length),
+// the per-run pass schedule and the event-time reductions.  Instantiated per
+// precision and spin in fgpe_ops_*.hip.
 #pragma once
 #include "fgpe_impl.hpp"
 
 namespace fgpe_impl {
 
-template <typename K> void allow_lds(K kernel, size_t bytes) {
-  if (bytes > 65536) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-// ------------------------------------------------------------- launches
-// Pass A over the z-planes [z0, z0 + zc) on stream st.  The chunk's lines
-// fill whole blocks (chunk_planes), so each block keeps its whole-grid index
-// and NORMC partial slot.
-template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
-  constexpr int Q = Cfg<T>::QX, P = N / Q;
-  constexpr int MS = 2 * SPIN + 1;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const int threads = P >= 256 ? P : 256;
-  const int lpw = threads / P;
-  const long lines0 = (long)z0 * h->ny, lines = (long)zc * h->ny;
-  const int blocks = (int)x_pass_blocks<T>(N, lines);
-  const int blk0 = (int)(lines0 / lpw);
-  const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
-  if (flags & XF_PHYS) {
-    int rc = ensure_partials(h, (size_t)x_pass_blocks<T>(N, (long)h->ny * h->nz) * MS);
-    if (rc) return rc;
-  }
-  // 3D: passes B and D apply the kx factor and 1/N
-  auto kern = h->dim == 3 ? k_xpass<T, SPIN, N, Q, false> : k_xpass<T, SPIN, N, Q, true>;
-  allow_lds(kern, lds);
-  prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
-                     flags, h->partials, blk0);
-  HIPCHK(hipGetLastError());
-  prof_end(h, (flags & XF_PHYS) ? FGPE_PASS_X : FGPE_PASS_X0);
-  return 0;
-}
-
-// NORMC partials of a whole pass A -> norms and the pending renormalisation
-template <typename T, int SPIN, int N> int launch_fin(fgpe_handle* h) {
-  constexpr int MS = 2 * SPIN + 1;
-  const int blocks = (int)x_pass_blocks<T>(N, (long)h->ny * h->nz);
-  prof_begin(h);
-  hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, blocks,
-                     finargs(h), h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
-  HIPCHK(hipGetLastError());
-  prof_end(h, FGPE_PASS_FIN);
-  return 0;
-}
-
-// y lines of one component (3D pass B) over x in [x0, x0 + xw) and z in
-// [z0, z0 + zc) on stream st.
+// pass launches, defined in fgpe_launch.hpp and instantiated per line length
+// N in fgpe_lines.hip
+template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc);
+template <typename T, int SPIN, int N> int launch_fin(fgpe_handle* h);
 template <typename T, int N>
-int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc) {
-  constexpr int Q = Cfg<T>::QY, P = N / Q, TX = Cfg<T>::TX;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "y pass needs %zu B of LDS", lds);
-  auto kern = k_ypass<T, N, Q, TX>;
-  allow_lds(kern, lds);
-  prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(xw / TX, zc, h->ms), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h),
-                     dir, scale, 1.0 / (double)h->npts, x0, z0);
-  HIPCHK(hipGetLastError());
-  prof_end(h, FGPE_PASS_Y);
-  return 0;
-}
+int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc);
+template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale);
+template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw);
+template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw);
+template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps);
+template <typename T, int SPIN, int DIM, int AXIS, int N>
+int launch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk);
 
-// 2D k pass (y lines, KE, SOC), all of the grid.
-template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
-  constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TXK;
-  const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX>();
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
-  auto kern = k_kpass<T, SPIN, N, Q, TX>;
-  allow_lds(kern, lds);
-  prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(h->nx / TX), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
-                     phys(h), scale);
-  HIPCHK(hipGetLastError());
-  prof_end(h, FGPE_PASS_K);
-  return 0;
-}
-
-// 3D pass C (z lines, KE) over x in [x0, x0 + xw) on stream st.
-template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw) {
-  constexpr int Q = Cfg<T>::QZ, P = N / Q, TX = Cfg<T>::TX;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "z pass needs %zu B of LDS", lds);
-  auto kern = k_zpass<T, N, Q, TX>;
-  allow_lds(kern, lds);
-  prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(xw / TX, h->ny, h->ms), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h),
-                     scale, x0);
-  HIPCHK(hipGetLastError());
-  prof_end(h, FGPE_PASS_Z);
-  return 0;
-}
-
-// 3D pass D (SOC, inverse y, 1/N) over x in [x0, x0 + xw) on stream st.
-template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
-  constexpr int Q = Cfg<T>::QK, P = N / Q, TX = Cfg<T>::TXK;
-  const size_t lds = dpass_lds_bytes<T, SPIN, N, Q, TX>();
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "d pass needs %zu B of LDS", lds);
-  auto kern = k_dpass<T, SPIN, N, Q, TX>;
-  allow_lds(kern, lds);
-  prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(xw / TX, h->nz), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
-                     1.0 / (double)h->npts, x0 / TX, 0);
-  HIPCHK(hipGetLastError());
-  prof_end(h, FGPE_PASS_K);
-  return 0;
-}
-
-template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps) {
-  constexpr int Q = Cfg<T>::Q1, P = N / Q;
-  constexpr int MS = 2 * SPIN + 1;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const size_t lds = TWS * sizeof(R_t<T>) + 16 * MS * sizeof(double) + 8 * sizeof(double) + line_stride(N) * sizeof(C<T>);
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "1D step needs %zu B of LDS", lds);
-  auto kern = k_line1d<T, SPIN, N, Q>;
-  allow_lds(kern, lds);
-  prof_begin(h);
-  hipLaunchKernelGGL(kern, dim3(1), dim3(P), lds, h->stream, (C<T>*)h->phi, tables<T>(h), phys(h), finargs(h),
-                     nsteps, h->dnorm, h->dsig, h->derr);
-  HIPCHK(hipGetLastError());
-  prof_end(h, FGPE_PASS_1D);
-  return 0;
-}
-
-// runtime size dispatch
+// runtime size dispatch over the line lengths of fgpe_impl.hpp
 template <typename T, int SPIN> int dispatch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
   switch (h->nx) {
-    case 32: return launch_x<T, SPIN, 32>(h, flags, st, z0, zc);
-    case 64: return launch_x<T, SPIN, 64>(h, flags, st, z0, zc);
-    case 128: return launch_x<T, SPIN, 128>(h, flags, st, z0, zc);
-    case 256: return launch_x<T, SPIN, 256>(h, flags, st, z0, zc);
-    case 512: return launch_x<T, SPIN, 512>(h, flags, st, z0, zc);
+#define FGPE_CASE(n) \
+  case n: return launch_x<T, SPIN, n>(h, flags, st, z0, zc);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported NX=%d", h->nx);
 }
@@ -149,64 +35,56 @@ template <typename T, int SPIN> int dispatch_x(fgpe_handle* h, int flags) {
 }
 template <typename T, int SPIN> int dispatch_fin(fgpe_handle* h) {
   switch (h->nx) {
-    case 32: return launch_fin<T, SPIN, 32>(h);
-    case 64: return launch_fin<T, SPIN, 64>(h);
-    case 128: return launch_fin<T, SPIN, 128>(h);
-    case 256: return launch_fin<T, SPIN, 256>(h);
-    case 512: return launch_fin<T, SPIN, 512>(h);
+#define FGPE_CASE(n) \
+  case n: return launch_fin<T, SPIN, n>(h);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported NX=%d", h->nx);
 }
 template <typename T>
 int dispatch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc) {
   switch (h->ny) {
-    case 32: return launch_y<T, 32>(h, dir, scale, st, x0, xw, z0, zc);
-    case 64: return launch_y<T, 64>(h, dir, scale, st, x0, xw, z0, zc);
-    case 128: return launch_y<T, 128>(h, dir, scale, st, x0, xw, z0, zc);
-    case 256: return launch_y<T, 256>(h, dir, scale, st, x0, xw, z0, zc);
-    case 512: return launch_y<T, 512>(h, dir, scale, st, x0, xw, z0, zc);
+#define FGPE_CASE(n) \
+  case n: return launch_y<T, n>(h, dir, scale, st, x0, xw, z0, zc);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
 }
 template <typename T, int SPIN> int dispatch_k(fgpe_handle* h, int scale) {
   switch (h->ny) {
-    case 32: return launch_k<T, SPIN, 32>(h, scale);
-    case 64: return launch_k<T, SPIN, 64>(h, scale);
-    case 128: return launch_k<T, SPIN, 128>(h, scale);
-    case 256: return launch_k<T, SPIN, 256>(h, scale);
-    case 512: return launch_k<T, SPIN, 512>(h, scale);
+#define FGPE_CASE(n) \
+  case n: return launch_k<T, SPIN, n>(h, scale);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
 }
 template <typename T> int dispatch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw) {
   switch (h->nz) {
-    case 32: return launch_z<T, 32>(h, scale, st, x0, xw);
-    case 64: return launch_z<T, 64>(h, scale, st, x0, xw);
-    case 128: return launch_z<T, 128>(h, scale, st, x0, xw);
-    case 256: return launch_z<T, 256>(h, scale, st, x0, xw);
-    case 512: return launch_z<T, 512>(h, scale, st, x0, xw);
+#define FGPE_CASE(n) \
+  case n: return launch_z<T, n>(h, scale, st, x0, xw);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported NZ=%d", h->nz);
 }
 template <typename T, int SPIN> int dispatch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
   switch (h->ny) {
-    case 32: return launch_d<T, SPIN, 32>(h, st, x0, xw);
-    case 64: return launch_d<T, SPIN, 64>(h, st, x0, xw);
-    case 128: return launch_d<T, SPIN, 128>(h, st, x0, xw);
-    case 256: return launch_d<T, SPIN, 256>(h, st, x0, xw);
-    case 512: return launch_d<T, SPIN, 512>(h, st, x0, xw);
+#define FGPE_CASE(n) \
+  case n: return launch_d<T, SPIN, n>(h, st, x0, xw);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported NY=%d", h->ny);
 }
 template <typename T, int SPIN> int dispatch_1d(fgpe_handle* h, long nsteps) {
   switch (h->nx) {
-    case 64: return launch_1d<T, SPIN, 64>(h, nsteps);
-    case 128: return launch_1d<T, SPIN, 128>(h, nsteps);
-    case 256: return launch_1d<T, SPIN, 256>(h, nsteps);
-    case 512: return launch_1d<T, SPIN, 512>(h, nsteps);
-    case 1024: return launch_1d<T, SPIN, 1024>(h, nsteps);
-    case 2048: return launch_1d<T, SPIN, 2048>(h, nsteps);
-    case 4096: return launch_1d<T, SPIN, 4096>(h, nsteps);
+#define FGPE_CASE(n) \
+  case n: return launch_1d<T, SPIN, n>(h, nsteps);
+    FGPE_1D_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported 1D NX=%d", h->nx);
 }
@@ -356,47 +234,14 @@ template <typename T> int normalize_t(fgpe_handle* h, double* tnorm) {
   return 0;
 }
 
-// Tiles of the observables line pass along `axis` (lines per tile from the
-// axis' line length), and the blocks that loop over them.
-template <typename T, int SPIN> long obs_axis_tiles(const fgpe_handle* h, int axis) {
-  const int n = axis == 0 ? h->nx : axis == 1 ? h->ny : h->nz;
-  const int L = n == 32    ? obs_lines<T, SPIN, 32>()
-                : n == 64  ? obs_lines<T, SPIN, 64>()
-                : n == 128 ? obs_lines<T, SPIN, 128>()
-                : n == 256 ? obs_lines<T, SPIN, 256>()
-                           : obs_lines<T, SPIN, 512>();
-  const long nbx = (h->nx + L - 1) / L;
-  return axis == 0 ? ((long)h->ny * h->nz + L - 1) / L : axis == 1 ? nbx * h->nz : nbx * h->ny;
-}
-template <typename T, int SPIN> long obs_axis_blocks(const fgpe_handle* h, int axis) {
-  return std::min(obs_axis_tiles<T, SPIN>(h, axis), OBS_MAX_BLOCKS);
-}
-
-// One axis of the 2D/3D observables (k_obs_line) for lines of N points;
-// returns the number of blocks launched, whose sums start at partials + off
-// (the caller has sized the partials for every axis).
-template <typename T, int SPIN, int DIM, int AXIS, int N>
-int launch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk) {
-  const size_t lds = obs_line_lds<T, SPIN, N>();
-  if (lds > h->lds_max) return fail(FGPE_EINVAL, "observables need %zu B of LDS", lds);
-  const long blocks = obs_axis_blocks<T, SPIN>(h, AXIS);
-  auto kern = k_obs_line<T, SPIN, DIM, AXIS, N>;
-  allow_lds(kern, lds);
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(OBS_TPB), lds, h->stream, (const C<T>*)h->phi, geom(h), a,
-                     obs_axis_tiles<T, SPIN>(h, AXIS), h->partials + off);
-  HIPCHK(hipGetLastError());
-  *nblk = (int)blocks;
-  return 0;
-}
 template <typename T, int SPIN, int DIM, int AXIS>
 int dispatch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk) {
   const int n = AXIS == 0 ? h->nx : AXIS == 1 ? h->ny : h->nz;
   switch (n) {
-    case 32: return launch_obs_axis<T, SPIN, DIM, AXIS, 32>(h, a, off, nblk);
-    case 64: return launch_obs_axis<T, SPIN, DIM, AXIS, 64>(h, a, off, nblk);
-    case 128: return launch_obs_axis<T, SPIN, DIM, AXIS, 128>(h, a, off, nblk);
-    case 256: return launch_obs_axis<T, SPIN, DIM, AXIS, 256>(h, a, off, nblk);
-    case 512: return launch_obs_axis<T, SPIN, DIM, AXIS, 512>(h, a, off, nblk);
+#define FGPE_CASE(n) \
+  case n: return launch_obs_axis<T, SPIN, DIM, AXIS, n>(h, a, off, nblk);
+    FGPE_LINE_SIZES(FGPE_CASE)
+#undef FGPE_CASE
   }
   return fail(FGPE_EINVAL, "unsupported line length %d", n);
 }

@@ -440,8 +440,9 @@ __global__ __launch_bounds__(OBS_TPB) void k_obs_line(const C_t<T>* __restrict__
   constexpr int L = obs_lines<T, SPIN, N>();
   constexpr int NS = N + 2;   // padded line stride
   constexpr int PTS = L * N;  // points of a tile
-  static_assert(PTS % OBS_TPB == 0, "observables block must tile its points");
-  constexpr int PER = PTS / OBS_TPB;
+  constexpr int PER = (PTS + OBS_TPB - 1) / OBS_TPB;
+  // with 3 * 2^a points per line the last round of a tile is partial
+  auto in_tile = [](int e) { return PTS % OBS_TPB == 0 || e < PTS; };
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);  // [MS][L][NS]
   static_assert(16 * NC * sizeof(double) <= (size_t)MS * L * NS * sizeof(C), "reduction must fit the tile");
@@ -482,13 +483,13 @@ __global__ __launch_bounds__(OBS_TPB) void k_obs_line(const C_t<T>* __restrict__
       for (int k = 0; k < PER; ++k) {
         const int e = threadIdx.x + k * OBS_TPB;
         const int l = AXIS == 0 ? e / N : e % L, i = AXIS == 0 ? e % N : e / L;
-        tmp[k] = valid(l) ? pm[gidx(l, i)] : mk<C>((T)0, (T)0);
+        tmp[k] = in_tile(e) && valid(l) ? pm[gidx(l, i)] : mk<C>((T)0, (T)0);
       }
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int e = threadIdx.x + k * OBS_TPB;
         const int l = AXIS == 0 ? e / N : e % L, i = AXIS == 0 ? e % N : e / L;
-        lds[(m * L + l) * NS + i] = tmp[k];
+        if (in_tile(e)) lds[(m * L + l) * NS + i] = tmp[k];
       }
     }
     __syncthreads();

@@ -64,13 +64,16 @@ PROGRAM CGPE
   IF (FEXIST) THEN
     OPEN (10, FILE=TRIM(INFILE), STATUS='OLD', FORM='FORMATTED', ACTION='READ')
   ELSE
-    WRITE (100,*) "'input", DIM, "D.dat' does not exist."
-    PRINT *, "'input", DIM, "D.dat' does not exist."
+    ! same text as SRC/cgpe3d.f90:228-229 (cgpe1d/cgpe2d name their own file),
+    ! one record whatever the line length of list-directed output
+    WRITE (100,'(1X,A)') "'input"//ACHAR(48 + DIM)//"D.dat' does not exist."
+    WRITE (*,'(1X,A)') "'input"//ACHAR(48 + DIM)//"D.dat' does not exist."
     STOP
   END IF
 
   ! ---- records of INPUT/inputND.dat (SRC/cgpe3d.f90:239-266)
   SWITCH_SOC = 0
+  IOSTAT = 0
   OPTION_FPC = 4
   MAG = 0.0D0
   DY = 1.0D0
@@ -189,6 +192,16 @@ PROGRAM CGPE
   IF (DIM < 3) Z = 0.0D0
   ALLOCATE (PHI(NX, NY, NZ, MS))
 
+  ! ---- real time needs initial_sol (SRC/cgpe3d.f90:383-390): checked after the
+  ! file1 header, as there, and before the device is touched
+  IF (SWITCH_IM /= 1 .AND. IOSTAT /= 0) THEN
+    WRITE (*,'(1X,A)') 'Check if the "initial_sol_spin'//SPIN_ID//'.dat" is available'
+    WRITE (*,'(1X,A)') 'if not, user can read the "solution_file_im_spin'//SPIN_ID//'.dat" file as an initial solution.'
+    WRITE (100,'(1X,A)') 'Check if the "initial_sol_spin'//SPIN_ID//'.dat" is available'
+    WRITE (100,'(1X,A)') 'if not, user can read the "solution_file_im_spin'//SPIN_ID//'.dat" file as an initial solution.'
+    STOP
+  END IF
+
   RC = FGPE_CREATE(PRM, H)
   IF (RC /= 0) CALL FAIL_STOP(RC)
   RC = FGPE_INITIAL_GUESS(H)
@@ -196,13 +209,6 @@ PROGRAM CGPE
 
   ! ---- real time: initial state from initial_sol (SRC/cgpe3d.f90:383-410)
   IF (SWITCH_IM /= 1) THEN
-    IF (IOSTAT /= 0) THEN
-      PRINT *, 'Check if the "initial_sol_spin'//SPIN_ID//'.dat" is available'
-      PRINT *, 'if not, user can read the "solution_file_im_spin'//SPIN_ID//'.dat" file as an initial solution.'
-      WRITE (100,*) 'Check if the "initial_sol_spin'//SPIN_ID//'.dat" is available'
-      WRITE (100,*) 'if not, user can read the "solution_file_im_spin'//SPIN_ID//'.dat" file as an initial solution.'
-      STOP
-    END IF
     SELECT CASE (DIM)
       CASE (1)
         DO I = 1, NX
@@ -344,7 +350,9 @@ CONTAINS
   SUBROUTINE FAIL_STOP(CODE)
     INTEGER(C_INT), INTENT(IN) :: CODE
     IF (CODE == FGPE_ENEWTON) THEN
-      WRITE (100,*) 'Newton-Raphason subroutine could not calculate the roots within', 1.0D-6, ' tolerance'
+      ! SRC/cgpe3d.f90:1556-1557, as one record
+      WRITE (100,'(1X,A,1X,ES22.15E3,A)') 'Newton-Raphason subroutine could not calculate the roots within', &
+            1.0D-6, ' tolerance'
     ELSE
       WRITE (100,'(A,I0,2A)') ' libfgpe error ', CODE, ': ', TRIM(FGPE_MESSAGE())
       WRITE (*,'(A,I0,2A)') ' libfgpe error ', CODE, ': ', TRIM(FGPE_MESSAGE())

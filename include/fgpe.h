@@ -72,7 +72,8 @@ typedef struct fgpe_input {
 typedef struct fgpe_params {
   int32_t dim;        /* 1, 2 or 3 */
   int32_t spin;       /* 1 or 2 */
-  int32_t n[3];       /* grid points per axis (unused axes 1); each even */
+  int32_t n[3];       /* grid points per axis (unused axes 1): 2^a or 3*2^a,
+                         32..512 in 2D/3D, 64..4096 in 1D */
   int32_t imag;       /* 1 imaginary time (CDT = -i DT), 0 real time (CDT = DT) */
   int32_t precision;  /* FGPE_FP64 or FGPE_FP32 */
   int32_t option_fpc; /* initial-guess selector used by fgpe_initial_guess */

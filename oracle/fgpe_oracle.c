@@ -982,7 +982,8 @@ static int newton(double* x, const double* n, double mag, double dt) {
     x[1] += p[1];
     if (fmax(fabs(p[0]), fabs(p[1])) <= TOL_NR) break;
   }
-  return fmax(fabs(p[0]), fabs(p[1])) > TOL_NR ? FGPE_ENEWTON : 0;
+  /* a NaN step fails too (the reference's .GT. test would let it through) */
+  return !(fabs(p[0]) <= TOL_NR && fabs(p[1]) <= TOL_NR) ? FGPE_ENEWTON : 0;
 }
 
 /* ------------------------------------------------------------------ step

@@ -58,6 +58,19 @@ CASES = {
                                natoms=100000, nu=(20, 20, 20)), 10),
     "2d_s2_nosoc_newton": (case(2, 2, (64, 64), (0.1, 0.1), 5e-4, (0.0, 0.0), mag=0.6, natoms=100000,
                                 nu=(20, 20, 400)), 20),
+    # grids of 3 * 2^a points per axis (radix-3 FFT stages), mixed with powers of two
+    "1d_s1_soc_imag_768": (case(1, 1, (768,), (0.04,), 5e-4, (0.5,)), 40),
+    "1d_s2_soc_real_192": (case(1, 2, (192,), (0.1,), 1.25e-4, (0.5,), imag=0, species=NA_SPIN2), 40),
+    "2d_s1_soc_imag_192x96": (case(2, 1, (192, 96), (0.1, 0.15), 1e-3, (0.7, 1.3), natoms=50000,
+                                   nu=(20, 25, 400)), 20),
+    # N = 3e4: at 1e5 this start amplifies a 1e-7 density perturbation 63x in
+    # 20 steps, so FP32 rounding inside the step, not the path, sets the error
+    "2d_s2_soc_real_96x192": (case(2, 2, (96, 192), (0.2, 0.1), 1e-3, (1.0, 0.5), imag=0, natoms=30000,
+                                   nu=(20, 20, 400)), 20),
+    "3d_s2_soc_real_96x64x32": (case(3, 2, (96, 64, 32), (0.3, 0.4, 0.8), 5e-4, (0.5, 0.3, 0.7), imag=0,
+                                     natoms=100000, nu=(20, 20, 20)), 10),
+    "3d_s1_soc_imag_32x96x192": (case(3, 1, (32, 96, 192), (0.8, 0.3, 0.15), 5e-4, (0.5, 0.5, 0.5),
+                                      natoms=100000, nu=(20, 20, 20)), 10),
 }
 
 
