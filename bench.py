@@ -118,6 +118,154 @@ def accuracy(args):
     print(json.dumps({"metric": "FP32 vs FP64 component densities, rel L2", "n": args.n, "rows": rows}), flush=True)
 
 
+def config1_input(imag, niter):
+    """q1D spin-1: NX=512, DX=0.05, DT=0.0005, 87Rb A0/A2 = 101.8/100.4 a_B,
+    NATOMS=1e4, trap 20/400/400 Hz, GAMMAX=0.5, TOL=1e-9, STP=100."""
+    from fortress_amd.abi import make_input
+    return make_input(1, 1, (512,), (0.05,), 0.0005, 87.0, (101.8, 100.4), 10000, (20.0, 400.0, 400.0), (0.5,),
+                      switch_im=imag, niter=niter, stp=100, tol=1e-9)
+
+
+def oracle_ground_state(O, inp):
+    """The oracle through the same events as fortress_amd.groundstate.solve
+    (the cgpe.x loop): (state, exit iteration, EN, CONV, CONV_EN)."""
+    p, _ = O.derive(inp)
+    o = O.Oracle(p)
+    o.initial_guess()
+    o.normt()
+    en_old = o.energy()[0]
+    stp, den = inp.stp, inp.stp * 2.0 * inp.dt
+    ii = 0
+    conv = conv_en = en = None
+    while ii < inp.niter:
+        o.run(stp - 1)
+        prev = o.get_phi()
+        o.run(1)
+        ii += stp
+        en = o.energy()[0]
+        conv = float(np.abs(o.get_phi() - prev).max() / den)
+        conv_en = abs(en - en_old) / den
+        if conv <= inp.tol or conv_en <= inp.tol:
+            break
+        en_old = en
+    phi = o.get_phi()
+    o.close()
+    return phi, ii, en, conv, conv_en
+
+
+def config1_bench(args):
+    """q1D spin-1 ground state in FP64 (cgpe.x's loop, NITER=20000, TOL=1e-9),
+    then 5000 real-time steps from it plus seeded 1% complex noise; the same
+    two runs by the FP64 oracle on the host are the CPU baseline and the
+    parity check."""
+    from fortress_amd import fgpe, groundstate
+    from fortress_amd.abi import FGPE_FP64
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as O
+    O.lib().or_set_threads(1)
+    inp = config1_input(1, args.config_niter)
+    # warm the device path once (module load, first-launch costs)
+    groundstate.solve(config1_input(1, 200), FGPE_FP64, keep_phi=False)
+    t0 = time.perf_counter()
+    g = groundstate.solve(inp, FGPE_FP64)
+    phi0 = noisy(g.phi, seed=4321)
+    p_re, _ = fgpe.derive(config1_input(0, 5000), FGPE_FP64)
+    with fgpe.Solver(p_re) as s:
+        s.upload(phi0)
+        s.run(args.config_rt_steps)
+        phi_rt = s.download()
+        en_rt = s.observables()[0]
+    gpu_s = time.perf_counter() - t0
+    # the oracle: same events, same noisy start
+    t0 = time.perf_counter()
+    ophi, oii, oen, oconv, oconv_en = oracle_ground_state(O, inp)
+    t_gs = time.perf_counter() - t0
+    p, _ = O.derive(config1_input(0, 5000))
+    o = O.Oracle(p)
+    o.set_phi(phi0)
+    t1 = time.perf_counter()
+    o.run(args.config_rt_steps)
+    t_rt = time.perf_counter() - t1
+    oen_rt = o.energy()[0]
+    ophi_rt = o.get_phi()
+    o.close()
+    cpu_s = time.perf_counter() - t0
+    rel = lambda a, b: float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+    line = {
+        "metric": "q1D spin-1 512 ground state (cgpe.x loop to TOL 1e-9 or NITER) + 5000 real-time steps, wall s",
+        "value": round(gpu_s, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp64",
+        "data": "synthetic: Gaussian guess; real time from the ground state + seeded 1% complex noise",
+        "config": {"workload": "q1D spin-1, NX=512, DX=0.05, DT=0.0005, 87Rb A0/A2=101.8/100.4, NATOMS=1e4, "
+                               "trap 20/400/400 Hz, GAMMAX=0.5, TOL=1e-9, STP=100, NITER=%d; then %d real-time "
+                               "steps" % (args.config_niter, args.config_rt_steps)},
+        "ground_state": {"converged": g.converged, "ii_print": g.ii_print, "en": g.en / 2.0, "conv": g.conv,
+                         "conv_en": g.conv_en, "wall_s": round(g.wall_s, 4),
+                         "steps_per_s": round(g.ii / g.wall_s, 1)},
+        "real_time": {"steps": args.config_rt_steps, "en": en_rt / 2.0},
+        "vs_oracle": {"ground_state_phi_rel_l2": rel(g.phi, ophi), "exit_ii": oii, "en_rel": abs(g.en - oen) / abs(oen),
+                      "conv_rel": abs(g.conv - oconv) / oconv, "real_time_density_rel_l2":
+                          rel(np.abs(phi_rt) ** 2, np.abs(ophi_rt) ** 2),
+                      "real_time_en_rel": abs(en_rt - oen_rt) / abs(oen_rt)},
+        "cpu_baseline": {"value": round(cpu_s, 3), "unit": "s", "cores": 1, "kind": "port",
+                         "sample": "the whole workload (ground state %.2f s + real time %.2f s) by the FP64 oracle "
+                                   "(oracle/fgpe_oracle.c), one thread" % (t_gs, t_rt)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def config2_bench(args):
+    """q2D spin-2 Rashba ground state to TOL=1e-9 (cgpe.x's loop, STP=100):
+    FP64 throughout, against FP32 propagation (FP64 reductions) finished in
+    FP64 once CONV reaches the FP32 floor (fortress_amd.groundstate); wall
+    clock, steps in each precision and the EN/MU/RMS error of the mixed run
+    against the FP64 one."""
+    from fortress_amd import groundstate
+    from fortress_amd.abi import FGPE_FP32, FGPE_FP64, make_input
+    from fortress_amd import fgpe
+    inp = make_input(2, 2, (256, 256), (0.1, 0.1), 0.001, 87.0, (87.93, 91.28, 99.18), 100000,
+                     (20.0, 20.0, 400.0), (1.0, 1.0), switch_im=1, niter=args.max_steps, stp=100, tol=1e-9)
+    # start: the Gaussian guess plus seeded 1% noise, so the branch of the
+    # spin texture is picked by the start and not by rounding
+    with fgpe.Solver(fgpe.derive(inp, FGPE_FP64)[0]) as s:
+        s.initial_guess()
+        s.normalize()
+        phi0 = noisy(s.download(), seed=777)
+    groundstate.solve(make_input(2, 2, (256, 256), (0.1, 0.1), 0.001, 87.0, (87.93, 91.28, 99.18), 100000,
+                                 (20.0, 20.0, 400.0), (1.0, 1.0), switch_im=1, niter=300, stp=100, tol=1e-9),
+                      FGPE_FP32, keep_phi=False)
+    logs = {}
+
+    def logger(tag):
+        def f(ii, conv, conv_en, prec):
+            if ii % 10000 == 0:
+                print("%s II=%d CONV=%.3e CONV_EN=%.3e %s" % (tag, ii, conv, conv_en,
+                                                             "fp32" if prec == FGPE_FP32 else "fp64"), flush=True)
+        return f
+    r64 = groundstate.solve(inp, FGPE_FP64, phi0=phi0, log=logger("fp64"))
+    rmx = groundstate.solve(inp, FGPE_FP32, phi0=phi0, finish=True, log=logger("fp32+fp64"))
+    r32 = groundstate.solve(inp, FGPE_FP32, phi0=phi0, finish=False, keep_phi=False,
+                            log=None) if args.config_fp32_only else None
+    rel = lambda a, b: float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b))))
+    line = {
+        "metric": "q2D spin-2 256^2 ground state to TOL 1e-9: wall s, FP32 bulk + FP64 finish",
+        "value": round(rmx.wall_s, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "fp32 propagation + fp64 finish, fp64 reductions",
+        "data": "synthetic: Gaussian guess + seeded 1% complex noise",
+        "config": {"workload": "q2D spin-2 Rashba, 256^2, DX=DY=0.1, DT=0.001, 87Rb A0/A2/A4=87.93/91.28/99.18, "
+                               "NATOMS=1e5, trap 20/20/400 Hz, GAMMAX=GAMMAY=1.0, TOL=1e-9, STP=100"},
+        "fp64": r64.as_dict(),
+        "fp32_fp64_finish": rmx.as_dict(),
+        "speedup_vs_fp64": round(r64.wall_s / rmx.wall_s, 3),
+        "error_vs_fp64": {"en_rel": abs(rmx.en - r64.en) / abs(r64.en), "mu_rel_max": rel(rmx.mu, r64.mu),
+                          "rms_rel_max": rel(rmx.rms, r64.rms), "mz_abs": abs(rmx.mz - r64.mz),
+                          "phi_density_rel_l2": float(np.linalg.norm(np.abs(rmx.phi) ** 2 - np.abs(r64.phi) ** 2)
+                                                      / np.linalg.norm(np.abs(r64.phi) ** 2))},
+    }
+    if r32 is not None:
+        line["fp32_only"] = r32.as_dict()
+    print(json.dumps(line), flush=True)
+
+
 def surrogate_bench(args):
     """Learned initial guess against the Gaussian guess: ground-state
     time-to-TOL of q2D spin-1 instances on the production grid for held-out
@@ -365,6 +513,13 @@ def main():
     ap.add_argument("--batch-solve", action="store_true",
                     help="with --batch: solve the sweep to --batch-tol in FP64 instead of timing steps")
     ap.add_argument("--batch-tol", type=float, default=1e-8)
+    ap.add_argument("--config1", action="store_true",
+                    help="instead of the 3D propagator, the q1D spin-1 ground state + real-time run against the oracle")
+    ap.add_argument("--config2", action="store_true",
+                    help="instead of the 3D propagator, the q2D spin-2 ground state to TOL: FP64 vs FP32 + FP64 finish")
+    ap.add_argument("--config-niter", type=int, default=20000)
+    ap.add_argument("--config-rt-steps", type=int, default=5000)
+    ap.add_argument("--config-fp32-only", action="store_true", help="with --config2: also FP32 without the finish")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -376,6 +531,10 @@ def main():
         return batch_bench(args)
     if args.surrogate:
         return surrogate_bench(args)
+    if args.config1:
+        return config1_bench(args)
+    if args.config2:
+        return config2_bench(args)
 
     from fortress_amd import fgpe
     from fortress_amd.abi import FGPE_FP32, FGPE_FP64
