@@ -237,7 +237,7 @@ def config2_bench(args):
 
     def logger(tag):
         def f(ii, conv, conv_en, prec):
-            if ii % 10000 == 0:
+            if ii % 50000 == 0:
                 print("%s II=%d CONV=%.3e CONV_EN=%.3e %s" % (tag, ii, conv, conv_en,
                                                              "fp32" if prec == FGPE_FP32 else "fp64"), flush=True)
         return f

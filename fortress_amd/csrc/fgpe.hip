@@ -87,8 +87,8 @@ template <typename T> int build_tables(fgpe_handle* h) {
   // the pass that transforms it: x by pass A (or the q1D kernel), y by pass B
   // in 3D and by the k-space pass in 2D, y again by the all-component pass D
   // in 3D (its own table, QK points per thread), z by pass C
-  const int qx = q_for(h->nx, h->dim == 1 ? Cfg<T>::Q1 : Cfg<T>::QX);
-  const int qy = q_for(h->ny, h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK);
+  const int qx = h->dim == 1 ? q_for(h->nx, Cfg<T>::Q1) : x_pass_q<T>(h->dim, h->nx);
+  const int qy = q_for(h->ny, h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK2);
   const int qz = q_for(h->nz, Cfg<T>::QZ);
   auto kx = kgrid(h->nx, p.d[0]), ky = kgrid(h->ny, p.d[1]), kz = kgrid(h->nz, p.d[2]);
   auto xx = xgrid(h->nx, p.d[0]), yy = xgrid(h->ny, p.d[1]), zz = xgrid(h->nz, p.d[2]);
@@ -555,8 +555,8 @@ int fgpe_run_noalloc(fgpe_handle* h, int64_t nsteps) {
   if (!ptr) return fail(FGPE_EDEVICE, "no-allocation scratch unavailable on device %d: %s", h->dev, why.c_str());
   // pass A writes MS partials per block (q1D keeps its norms inside its one block)
   const size_t need = h->dim == 1 ? 0
-                      : (size_t)h->ms * (size_t)(h->fp32 ? x_pass_blocks<float>(h->nx, (long)h->ny * h->nz)
-                                                         : x_pass_blocks<double>(h->nx, (long)h->ny * h->nz));
+                      : (size_t)h->ms * (size_t)(h->fp32 ? x_pass_blocks<float>(h->dim, h->nx, (long)h->ny * h->nz)
+                                                         : x_pass_blocks<double>(h->dim, h->nx, (long)h->ny * h->nz));
   if (need > NOALLOC_SCRATCH_DOUBLES)
     return fail(FGPE_EINVAL, "grid %dx%dx%d needs %zu doubles of scratch; the no-allocation limit is %zu", h->nx,
                 h->ny, h->nz, need, NOALLOC_SCRATCH_DOUBLES);

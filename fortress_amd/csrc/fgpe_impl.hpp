@@ -46,6 +46,13 @@ template <> struct Cfg<float> {
   // x columns per pass-C block: 256 threads, so three blocks share a CU and
   // their load, transform and store phases overlap
   static constexpr int TXK = 8;
+  // q2D: a grid has only NY x lines, so pass A runs one line per block to
+  // spread over the CUs, at 4 points per thread (radix-4 stages: half the
+  // roundings of radix 2); the k pass keeps QK and TXK (narrower tiles were
+  // slower in FP32)
+  static constexpr int QX2 = 4;
+  static constexpr int QK2 = 8;
+  static constexpr int TXK2 = 8;
 };
 template <> struct Cfg<double> {
   static constexpr int QX = 8;
@@ -55,6 +62,12 @@ template <> struct Cfg<double> {
   static constexpr int Q1 = 8;
   static constexpr int TX = 8;
   static constexpr int TXK = 8;
+  // q2D: pass A at 2 points per thread, one line per block; the k pass in
+  // tiles of 2 columns (256^2 spin-2: 13.0k -> 21.3k steps/s; 2 points per
+  // thread in the k pass was slower)
+  static constexpr int QX2 = 2;
+  static constexpr int QK2 = 4;
+  static constexpr int TXK2 = 2;
 };
 
 // Line lengths with compiled passes: 2D/3D axes, and the q1D line.  A
@@ -68,14 +81,20 @@ template <> struct Cfg<double> {
 // factor has a stage (12 where the default is 16, else 6).
 __host__ __device__ constexpr int q_for(int n, int qdef) { return (n & (n - 1)) == 0 ? qdef : (qdef >= 12 ? 12 : 6); }
 
-// Blocks of one pass-A launch: P = NX/QX threads per x line, lines packed into
-// blocks of at least 256 threads.  Each block writes MS NORMC partials.
-template <typename T> inline int x_pass_lpw(int nx) {
-  const int p = nx / q_for(nx, Cfg<T>::QX);
+// Points per thread of pass A: QX in 3D, QX2 in q2D.
+template <typename T> constexpr int x_pass_q(int dim, int nx) {
+  return q_for(nx, dim == 2 ? Cfg<T>::QX2 : Cfg<T>::QX);
+}
+// Blocks of one pass-A launch: P = NX/Q threads per x line, lines packed into
+// blocks of at least 256 threads (3D) or of one wave (q2D, where the grid has
+// few lines).  Each block writes MS NORMC partials.
+template <typename T> inline int x_pass_lpw(int dim, int nx) {
+  const int p = nx / x_pass_q<T>(dim, nx);
+  if (dim == 2) return p >= 64 ? 1 : 64 / p;
   return (p >= 256 ? p : 256) / p;
 }
-template <typename T> inline long x_pass_blocks(int nx, long nlines) {
-  const int lpw = x_pass_lpw<T>(nx);
+template <typename T> inline long x_pass_blocks(int dim, int nx, long nlines) {
+  const int lpw = x_pass_lpw<T>(dim, nx);
   return (nlines + lpw - 1) / lpw;
 }
 
@@ -284,7 +303,7 @@ template <typename T> inline fgpe::Tables<T> tables(fgpe_handle* h) {
 template <typename T> inline int chunk_planes(const fgpe_handle* h) {
   const int zc = h->zchunk;
   if (zc <= 0 || zc >= h->nz || h->nz % zc) return h->nz;
-  if (((long)zc * h->ny) % x_pass_lpw<T>(h->nx)) return h->nz;
+  if (((long)zc * h->ny) % x_pass_lpw<T>(h->dim, h->nx)) return h->nz;
   return zc;
 }
 // x-columns per chunk of the second phase: the handle's choice when it

@@ -13,26 +13,27 @@ template <typename K> void allow_lds(K kernel, size_t bytes) {
 }
 
 // ------------------------------------------------------------- launches
-// Pass A over the z-planes [z0, z0 + zc) on stream st.  The chunk's lines
-// fill whole blocks (chunk_planes), so each block keeps its whole-grid index
-// and NORMC partial slot.
-template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
-  constexpr int Q = q_for(N, Cfg<T>::QX), P = N / Q;
+// Pass A over the z-planes [z0, z0 + zc) on stream st, Q points per thread.
+// The chunk's lines fill whole blocks (chunk_planes), so each block keeps its
+// whole-grid index and NORMC partial slot.  FULL: q2D (the pass also applies
+// 1/N and the kx factor; in 3D passes B and D do).
+template <typename T, int SPIN, int N, int Q, bool FULL>
+int launch_x_q(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
+  constexpr int P = N / Q;
   constexpr int MS = 2 * SPIN + 1;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const int threads = P >= 256 ? P : 256;
-  const int lpw = threads / P;
+  const int lpw = x_pass_lpw<T>(h->dim, N);
+  const int threads = lpw * P;
   const long lines0 = (long)z0 * h->ny, lines = (long)zc * h->ny;
-  const int blocks = (int)x_pass_blocks<T>(N, lines);
+  const int blocks = (int)x_pass_blocks<T>(h->dim, N, lines);
   const int blk0 = (int)(lines0 / lpw);
   const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
   if (flags & XF_PHYS) {
-    int rc = ensure_partials(h, (size_t)x_pass_blocks<T>(N, (long)h->ny * h->nz) * MS);
+    int rc = ensure_partials(h, (size_t)x_pass_blocks<T>(h->dim, N, (long)h->ny * h->nz) * MS);
     if (rc) return rc;
   }
-  // 3D: passes B and D apply the kx factor and 1/N
-  auto kern = h->dim == 3 ? k_xpass<T, SPIN, N, Q, false> : k_xpass<T, SPIN, N, Q, true>;
+  auto kern = k_xpass<T, SPIN, N, Q, FULL>;
   allow_lds(kern, lds);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
@@ -41,11 +42,15 @@ template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags, h
   prof_end(h, (flags & XF_PHYS) ? FGPE_PASS_X : FGPE_PASS_X0);
   return 0;
 }
+template <typename T, int SPIN, int N> int launch_x(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
+  if (h->dim == 3) return launch_x_q<T, SPIN, N, q_for(N, Cfg<T>::QX), false>(h, flags, st, z0, zc);
+  return launch_x_q<T, SPIN, N, q_for(N, Cfg<T>::QX2), true>(h, flags, st, z0, zc);
+}
 
 // NORMC partials of a whole pass A -> norms and the pending renormalisation
 template <typename T, int SPIN, int N> int launch_fin(fgpe_handle* h) {
   constexpr int MS = 2 * SPIN + 1;
-  const int blocks = (int)x_pass_blocks<T>(N, (long)h->ny * h->nz);
+  const int blocks = (int)x_pass_blocks<T>(h->dim, N, (long)h->ny * h->nz);
   prof_begin(h);
   hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, blocks,
                      finargs(h), h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
@@ -74,7 +79,7 @@ int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw,
 
 // 2D k pass (y lines, KE, SOC), all of the grid.
 template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
-  constexpr int Q = q_for(N, Cfg<T>::QK), P = N / Q, TX = Cfg<T>::TXK;
+  constexpr int Q = q_for(N, Cfg<T>::QK2), P = N / Q, TX = Cfg<T>::TXK2;
   const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
   auto kern = k_kpass<T, SPIN, N, Q, TX>;

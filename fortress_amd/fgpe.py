@@ -38,6 +38,12 @@ EXPORTS = (
     "fgpe_conv_max",
     "fgpe_center_abs",
     "fgpe_sync",
+    "fgpe_checkpoint_save",
+    "fgpe_checkpoint_load",
+    "fgpe_checkpoint_info",
+    "fgpe_write_dump",
+    "fgpe_format_f",
+    "fgpe_read_initial_sol",
     "fgpe_last_run_ms",
     "fgpe_pass_times",
     "fgpe_last_error",
@@ -74,6 +80,17 @@ class BatchResult(C.Structure):
                 ("mu", C.c_double * 5), ("rms", C.c_double * 5), ("norm", C.c_double * 5)]
 
 
+class LoopState(C.Structure):
+    """fgpe_loop_state: the caller's loop state stored in a checkpoint."""
+    _fields_ = [("ii", C.c_int64), ("en_old", C.c_double), ("user", C.c_double * 6)]
+
+
+class CheckpointInfo(C.Structure):
+    _fields_ = [("version", C.c_int32), ("dim", C.c_int32), ("spin", C.c_int32), ("precision", C.c_int32),
+                ("imag", C.c_int32), ("n", C.c_int32 * 3), ("dt", C.c_double), ("ii", C.c_int64),
+                ("en_old", C.c_double), ("payload_bytes", C.c_int64)]
+
+
 class FgpeError(RuntimeError):
     pass
 
@@ -105,6 +122,9 @@ def lib():
         L.fgpe_conv_max.argtypes = [vp, dp]
         L.fgpe_center_abs.argtypes = [vp, dp]
         L.fgpe_last_run_ms.argtypes = [vp, dp]
+        L.fgpe_checkpoint_save.argtypes = [vp, C.c_char_p, C.POINTER(LoopState)]
+        L.fgpe_checkpoint_load.argtypes = [vp, C.c_char_p, C.POINTER(LoopState)]
+        L.fgpe_checkpoint_info.argtypes = [C.c_char_p, C.POINTER(CheckpointInfo)]
         L.fgpe_pass_times.argtypes = [vp, i64, dp, C.POINTER(C.c_int32)]
         L.fgpe_last_error.argtypes = [C.c_char_p, C.c_size_t]
         L.fgpe_batch_create.argtypes = [C.POINTER(FgpeParams), i32, C.POINTER(vp)]
@@ -124,6 +144,14 @@ def lib():
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
+
+
+def checkpoint_info(path):
+    """Header of a checkpoint file, its checksum verified (no device needed)."""
+    info = CheckpointInfo()
+    _check(lib().fgpe_checkpoint_info(os.fsencode(path), C.byref(info)), "fgpe_checkpoint_info")
+    return dict(version=info.version, dim=info.dim, spin=info.spin, precision=info.precision, imag=info.imag,
+                n=tuple(info.n), dt=info.dt, ii=info.ii, en_old=info.en_old, payload_bytes=info.payload_bytes)
 
 
 def last_error():
@@ -254,6 +282,21 @@ class Solver:
 
     def sync(self):
         _check(lib().fgpe_sync(self.h), "fgpe_sync")
+
+    def checkpoint_save(self, path, ii=0, en_old=0.0, user=()):
+        """Binary checkpoint: state as held on the device, device scalars and
+        the loop state (ii, en_old, up to six user values)."""
+        st = LoopState(ii=int(ii), en_old=float(en_old))
+        for i, v in enumerate(user):
+            st.user[i] = v
+        _check(lib().fgpe_checkpoint_save(self.h, os.fsencode(path), C.byref(st)), "fgpe_checkpoint_save")
+
+    def checkpoint_load(self, path):
+        """Resume from a checkpoint of a run with the same parameters:
+        (ii, en_old, user values)."""
+        st = LoopState()
+        _check(lib().fgpe_checkpoint_load(self.h, os.fsencode(path), C.byref(st)), "fgpe_checkpoint_load")
+        return st.ii, st.en_old, tuple(st.user)
 
     PASS_NAMES = ("x", "x0", "y", "k", "finalize", "line1d", "z")
 

@@ -10,6 +10,15 @@
 ! Usage:  cgpe.x [1D|2D|3D] [fp64|fp32]     (default 3D fp64, like the
 ! reference Makefile's default DIMENSION=3D; fp32 selects FP32 propagation with
 ! FP64 reductions).  The environment variable FGPE_PRECISION=fp32 does the same.
+!
+! Binary checkpoints (opt-in, no counterpart in the reference; without these
+! variables every file is as the reference writes it):
+!   FGPE_CHECKPOINT=path, FGPE_CHECKPOINT_EVERY=k   write a checkpoint of the
+!     run (state, Newton state, II, EN_OLD) to path every k steps;
+!   FGPE_RESUME=path   start from such a checkpoint of a run with the same
+!     input (imaginary or real time; initial_sol is then not read) at its II:
+!     the files then hold the rows from there on, and the run continues
+!     bit-identically to one that never stopped (given the same event steps).
 PROGRAM CGPE
   USE, INTRINSIC :: ISO_C_BINDING
   USE FGPE
@@ -23,7 +32,7 @@ PROGRAM CGPE
   INTEGER :: NATOMS, I, J, K, L, II, IIPRINT, IOSTAT, NEXTEV, RC, NX2, NY2, NZ2
   REAL(8) :: DX, DY, DZ, DT, M, A0, A2, A4, NUX, NUY, NUZ, TOL, GAMMAX, GAMMAY, GAMMAZ, MAG
   REAL(8) :: AOSC, TNORM, EN, EN_OLD, MZ, CONV, CONV_EN, DMAX, START, FINISH
-  REAL(8) :: NORM(5), RMS(5), MU(5), CEN(5), TMP(10)
+  REAL(8) :: NORM(5), RMS(5), MU(5), CEN(5)
   REAL(8), ALLOCATABLE :: X(:), Y(:), Z(:)
   COMPLEX(8), ALLOCATABLE :: PHI(:,:,:,:)
   CHARACTER(LEN=1) :: SPIN_ID
@@ -33,7 +42,11 @@ PROGRAM CGPE
   CHARACTER(LEN=10) :: DATE, TIME, ZONE
   INTEGER :: VALUES(8), COUNT0, COUNT1, RATE
   INTEGER(C_INT32_T) :: PREC
-  LOGICAL :: FEXIST, EVSTP, EVNSTP
+  LOGICAL :: FEXIST, EVSTP, EVNSTP, RESUMED
+  CHARACTER(LEN=1024) :: CKPATH, RSPATH
+  CHARACTER(LEN=32) :: EVERYSTR
+  INTEGER :: CKEVERY
+  TYPE(FGPE_LOOP_STATE) :: LS
 
   CALL SYSTEM_CLOCK(COUNT0, RATE)
   CALL DATE_AND_TIME(DATE, TIME, ZONE, VALUES)
@@ -42,6 +55,16 @@ PROGRAM CGPE
   PREC = FGPE_FP64
   CALL GET_ENVIRONMENT_VARIABLE('FGPE_PRECISION', ARG)
   IF (TRIM(ARG) == 'fp32' .OR. TRIM(ARG) == 'FP32') PREC = FGPE_FP32
+  CKPATH = ' '
+  RSPATH = ' '
+  EVERYSTR = ' '
+  CKEVERY = 0
+  CALL GET_ENVIRONMENT_VARIABLE('FGPE_CHECKPOINT', CKPATH)
+  CALL GET_ENVIRONMENT_VARIABLE('FGPE_CHECKPOINT_EVERY', EVERYSTR)
+  CALL GET_ENVIRONMENT_VARIABLE('FGPE_RESUME', RSPATH)
+  IF (LEN_TRIM(EVERYSTR) > 0) READ (EVERYSTR,*) CKEVERY
+  IF (LEN_TRIM(CKPATH) == 0) CKEVERY = 0
+  RESUMED = LEN_TRIM(RSPATH) > 0
   DO I = 1, COMMAND_ARGUMENT_COUNT()
     CALL GET_COMMAND_ARGUMENT(I, ARG)
     SELECT CASE (TRIM(ARG))
@@ -194,7 +217,7 @@ PROGRAM CGPE
 
   ! ---- real time needs initial_sol (SRC/cgpe3d.f90:383-390): checked after the
   ! file1 header, as there, and before the device is touched
-  IF (SWITCH_IM /= 1 .AND. IOSTAT /= 0) THEN
+  IF (SWITCH_IM /= 1 .AND. IOSTAT /= 0 .AND. .NOT. RESUMED) THEN
     WRITE (*,'(1X,A)') 'Check if the "initial_sol_spin'//SPIN_ID//'.dat" is available'
     WRITE (*,'(1X,A)') 'if not, user can read the "solution_file_im_spin'//SPIN_ID//'.dat" file as an initial solution.'
     WRITE (100,'(1X,A)') 'Check if the "initial_sol_spin'//SPIN_ID//'.dat" is available'
@@ -208,66 +231,53 @@ PROGRAM CGPE
   IF (RC /= 0) CALL FAIL_STOP(RC)
 
   ! ---- real time: initial state from initial_sol (SRC/cgpe3d.f90:383-410)
-  IF (SWITCH_IM /= 1) THEN
-    SELECT CASE (DIM)
-      CASE (1)
-        DO I = 1, NX
-          READ (1,*) X(I), (TMP(L), L = 1, 2*MS)
-          DO L = 1, MS
-            PHI(I,1,1,L) = SQRT(TMP(2*L-1))*EXP(CMPLX(0.0D0, TMP(2*L), KIND=8))
-          END DO
-        END DO
-      CASE (2)
-        DO I = 1, NX
-          DO J = 1, NY
-            READ (1,*) X(I), Y(J), (TMP(L), L = 1, 2*MS)
-            DO L = 1, MS
-              PHI(I,J,1,L) = SQRT(TMP(2*L-1))*EXP(CMPLX(0.0D0, TMP(2*L), KIND=8))
-            END DO
-          END DO
-        END DO
-      CASE (3)
-        DO I = 1, NX
-          DO J = 1, NY
-            DO K = 1, NZ
-              READ (1,*) (TMP(L), L = 1, 2*MS)
-              DO L = 1, MS
-                PHI(I,J,K,L) = SQRT(TMP(2*L-1))*EXP(CMPLX(0.0D0, TMP(2*L), KIND=8))
-              END DO
-            END DO
-          END DO
-        END DO
-    END SELECT
+  IF (SWITCH_IM /= 1 .AND. .NOT. RESUMED) THEN
+    ! SRC/cgpe3d.f90:391-409 (2D SRC/cgpe2d.f90:382-397, 1D SRC/cgpe1d.f90:
+    ! 377-388), parsed on the input's thread count: X (and Y) come from the file
     CLOSE (1)
+    RC = FGPE_READ_INITIAL_SOL('./INPUT/initial_sol_spin'//SPIN_ID//'.dat'//C_NULL_CHAR, INT(DIM, C_INT32_T), &
+                               INP%N, INT(MS, C_INT32_T), PHI, X, Y, INT(THREADS, C_INT32_T))
+    IF (RC /= 0) CALL FAIL_STOP(RC)
     RC = FGPE_UPLOAD(H, PHI)
     IF (RC /= 0) CALL FAIL_STOP(RC)
   END IF
 
-  IF (SWITCH_IM == 1) THEN
-    RC = FGPE_NORMALIZE(H, TNORM)
+  IF (RESUMED) THEN
+    ! ---- resume: state, Newton state, II and EN_OLD from the checkpoint
+    RC = FGPE_CHECKPOINT_LOAD(H, TRIM(RSPATH)//C_NULL_CHAR, LS)
     IF (RC /= 0) CALL FAIL_STOP(RC)
-  END IF
+    II = INT(LS%II)
+    EN_OLD = LS%EN_OLD
+    RC = FGPE_LAST_NORMS(H, NORM)
+    IF (RC /= 0) CALL FAIL_STOP(RC)
+  ELSE
+    IF (SWITCH_IM == 1) THEN
+      RC = FGPE_NORMALIZE(H, TNORM)
+      IF (RC /= 0) CALL FAIL_STOP(RC)
+    END IF
 
-  ! ---- initial observables (SRC/cgpe3d.f90:416-431)
-  RC = FGPE_NORMC(H, NORM)
-  IF (RC /= 0) CALL FAIL_STOP(RC)
-  RC = FGPE_OBSERVABLES(H, EN, MU, MZ, RMS)
-  IF (RC /= 0) CALL FAIL_STOP(RC)
-  RC = FGPE_CENTER_ABS(H, CEN)
-  IF (RC /= 0) CALL FAIL_STOP(RC)
-  CALL ROW_INITIAL()
-  EN_OLD = 0.0D0
-  II = 0
-  WRITE (3,'(F14.6,6F18.10)') 2.0D0*II*DT, EN/2.0D0, (RMS(L), L = 1, MS)
-  WRITE (4,'(F14.6,7F18.10)') 2.0D0*II*DT, (NORM(L), L = 1, MS), SUM(NORM(1:MS)), MZ
-  ! EN_OLD starts at 0 (SRC/cgpe3d.f90:307) and becomes EN at the end of every
-  ! step (529), so the first CONV_EN test sees the initial EN unless STP = 1
-  IF (STP > 1) EN_OLD = EN
+    ! ---- initial observables (SRC/cgpe3d.f90:416-431)
+    RC = FGPE_NORMC(H, NORM)
+    IF (RC /= 0) CALL FAIL_STOP(RC)
+    RC = FGPE_OBSERVABLES(H, EN, MU, MZ, RMS)
+    IF (RC /= 0) CALL FAIL_STOP(RC)
+    RC = FGPE_CENTER_ABS(H, CEN)
+    IF (RC /= 0) CALL FAIL_STOP(RC)
+    CALL ROW_INITIAL()
+    EN_OLD = 0.0D0
+    II = 0
+    WRITE (3,'(F14.6,6F18.10)') 2.0D0*II*DT, EN/2.0D0, (RMS(L), L = 1, MS)
+    WRITE (4,'(F14.6,7F18.10)') 2.0D0*II*DT, (NORM(L), L = 1, MS), SUM(NORM(1:MS)), MZ
+    ! EN_OLD starts at 0 (SRC/cgpe3d.f90:307) and becomes EN at the end of every
+    ! step (529), so the first CONV_EN test sees the initial EN unless STP = 1
+    IF (STP > 1) EN_OLD = EN
+  END IF
 
   ! ---- time loop, advanced between events (SRC/cgpe3d.f90:451-567)
   IIPRINT = NITER
   DO WHILE (II < NITER)
     NEXTEV = MIN(NITER, (II/STP + 1)*STP, (II/NSTP + 1)*NSTP)
+    IF (CKEVERY > 0) NEXTEV = MIN(NEXTEV, (II/CKEVERY + 1)*CKEVERY)
     EVSTP = MOD(NEXTEV, STP) == 0
     EVNSTP = MOD(NEXTEV, NSTP) == 0
     IF (SWITCH_IM == 1 .AND. EVSTP) THEN
@@ -314,6 +324,15 @@ PROGRAM CGPE
       IF (RC /= 0) CALL FAIL_STOP(RC)
       CALL DUMP_TMP()
     END IF
+    IF (CKEVERY > 0) THEN
+      IF (MOD(II, CKEVERY) == 0) THEN
+        LS%II = II
+        LS%EN_OLD = EN_OLD
+        LS%USER = 0.0D0
+        RC = FGPE_CHECKPOINT_SAVE(H, TRIM(CKPATH)//C_NULL_CHAR, LS)
+        IF (RC /= 0) CALL FAIL_STOP(RC)
+      END IF
+    END IF
   END DO
 
   ! ---- final observables and solution (SRC/cgpe3d.f90:569-602)
@@ -324,8 +343,10 @@ PROGRAM CGPE
   CALL ROW_FINAL(IIPRINT)
   RC = FGPE_DOWNLOAD(H, PHI)
   IF (RC /= 0) CALL FAIL_STOP(RC)
-  CALL DUMP_SOLUTION()
+  ! unit 8 (opened at the start, as in the reference) is closed before the
+  ! file is written through the library
   CLOSE (8)
+  CALL DUMP_SOLUTION()
   CALL FGPE_DESTROY(H)
   PRINT *, 'Program execution complete'
   CALL DATE_AND_TIME(DATE, TIME, ZONE, VALUES)
@@ -503,55 +524,19 @@ CONTAINS
   ! transient dumps every NSTP: 1D SRC/cgpe1d.f90:510-518, 2D
   ! SRC/cgpe2d.f90:519-530, 3D (full, xy and xz slices) SRC/cgpe3d.f90:532-566
   SUBROUTINE DUMP_TMP()
-    OPEN (7, FILE='./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'.dat', STATUS='replace', FORM='FORMATTED', &
-          ACTION='WRITE')
+    ! NSTP dumps: 1D SRC/cgpe1d.f90:513-516, 2D SRC/cgpe2d.f90:521-528,
+    ! 3D SRC/cgpe3d.f90:534-563 (full grid, then the z = NZ/2+1 and y = NY/2+1
+    ! slices), formatted on the input's thread count
     SELECT CASE (DIM)
       CASE (1)
-        DO I = 1, NX
-          WRITE (7,'(F11.5,10F14.6)') X(I), (ABS(PHI(I,1,1,L))**2, &
-                 ATAN2(AIMAG(PHI(I,1,1,L)), REAL(PHI(I,1,1,L))), L = 1, MS)
-        END DO
+        CALL DUMP('./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'.dat', FGPE_DUMP_1D_TMP)
       CASE (2)
-        DO I = 1, NX
-          DO J = 1, NY
-            WRITE (7,'(2F8.2,10F10.5)') X(I), Y(J), (ABS(PHI(I,J,1,L))**2, &
-                   ATAN2(AIMAG(PHI(I,J,1,L)), REAL(PHI(I,J,1,L))), L = 1, MS)
-          END DO
-          WRITE (7,'(11F10.2)')
-        END DO
+        CALL DUMP('./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'.dat', FGPE_DUMP_2D)
       CASE (3)
-        DO I = 1, NX
-          DO J = 1, NY
-            DO K = 1, NZ
-              WRITE (7,'(10F16.10)') (ABS(PHI(I,J,K,L))**2, ATAN2(AIMAG(PHI(I,J,K,L)), REAL(PHI(I,J,K,L))), &
-                     L = 1, MS)
-            END DO
-          END DO
-        END DO
+        CALL DUMP('./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'.dat', FGPE_DUMP_3D)
+        CALL DUMP('./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'_xy.dat', FGPE_DUMP_3D_XY)
+        CALL DUMP('./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'_xz.dat', FGPE_DUMP_3D_XZ)
     END SELECT
-    CLOSE (7)
-    IF (DIM == 3) THEN
-      OPEN (11, FILE='./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'_xy.dat', STATUS='replace', FORM='FORMATTED', &
-            ACTION='WRITE')
-      DO I = 1, NX
-        DO J = 1, NY
-          WRITE (11,'(2F10.4,10F14.8)') X(I), Y(J), (ABS(PHI(I,J,NZ2,L))**2, &
-                 ATAN2(AIMAG(PHI(I,J,NZ2,L)), REAL(PHI(I,J,NZ2,L))), L = 1, MS)
-        END DO
-        WRITE (11,'(2F10.4,10F14.8)')
-      END DO
-      CLOSE (11)
-      OPEN (12, FILE='./OUTPUT/tmp_solution_file_spin'//SPIN_ID//'_xz.dat', STATUS='replace', FORM='FORMATTED', &
-            ACTION='WRITE')
-      DO I = 1, NX
-        DO J = 1, NZ
-          WRITE (12,'(2F10.4,10F14.8)') X(I), Z(J), (ABS(PHI(I,NY2,J,L))**2, &
-                 ATAN2(AIMAG(PHI(I,NY2,J,L)), REAL(PHI(I,NY2,J,L))), L = 1, MS)
-        END DO
-        WRITE (12,'(2F10.4,10F14.8)')
-      END DO
-      CLOSE (12)
-    END IF
   END SUBROUTINE DUMP_TMP
 
   ! final solution: 1D SRC/cgpe1d.f90:547-549, 2D SRC/cgpe2d.f90:560-565,
@@ -559,27 +544,19 @@ CONTAINS
   SUBROUTINE DUMP_SOLUTION()
     SELECT CASE (DIM)
       CASE (1)
-        DO I = 1, NX
-          WRITE (8,'(F11.5,10F18.10)') X(I), (ABS(PHI(I,1,1,L))**2, &
-                 ATAN2(AIMAG(PHI(I,1,1,L)), REAL(PHI(I,1,1,L))), L = 1, MS)
-        END DO
+        CALL DUMP('./OUTPUT/solution_file_'//TAG//'_spin'//SPIN_ID//'.dat', FGPE_DUMP_1D_SOL)
       CASE (2)
-        DO I = 1, NX
-          DO J = 1, NY
-            WRITE (8,'(2F8.2,10F10.5)') X(I), Y(J), (ABS(PHI(I,J,1,L))**2, &
-                   ATAN2(AIMAG(PHI(I,J,1,L)), REAL(PHI(I,J,1,L))), L = 1, MS)
-          END DO
-          WRITE (8,'(2F8.2,10F10.5)')
-        END DO
+        CALL DUMP('./OUTPUT/solution_file_'//TAG//'_spin'//SPIN_ID//'.dat', FGPE_DUMP_2D)
       CASE (3)
-        DO I = 1, NX
-          DO J = 1, NY
-            DO K = 1, NZ
-              WRITE (8,'(10F16.10)') (ABS(PHI(I,J,K,L))**2, ATAN2(AIMAG(PHI(I,J,K,L)), REAL(PHI(I,J,K,L))), &
-                     L = 1, MS)
-            END DO
-          END DO
-        END DO
+        CALL DUMP('./OUTPUT/solution_file_'//TAG//'_spin'//SPIN_ID//'.dat', FGPE_DUMP_3D)
     END SELECT
   END SUBROUTINE DUMP_SOLUTION
+
+  SUBROUTINE DUMP(PATH, KIND)
+    CHARACTER(LEN=*), INTENT(IN) :: PATH
+    INTEGER(C_INT32_T), INTENT(IN) :: KIND
+    RC = FGPE_WRITE_DUMP(PATH//C_NULL_CHAR, KIND, INP%N, INT(MS, C_INT32_T), PHI, X, Y, Z, &
+                         INT(THREADS, C_INT32_T))
+    IF (RC /= 0) CALL FAIL_STOP(RC)
+  END SUBROUTINE DUMP
 END PROGRAM CGPE

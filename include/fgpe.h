@@ -186,6 +186,71 @@ int fgpe_center_abs(fgpe_handle* h, double* absphi);
 /* Wait for all work queued on the handle's stream. */
 int fgpe_sync(fgpe_handle* h);
 
+/* Binary checkpoint and resume (no counterpart in the reference, which can
+ * restart only real time, from the 10-decimal text file initial_sol:
+ * SRC/cgpe3d.f90:383-410, SRC/cgpe2d.f90:378-400, SRC/cgpe1d.f90:372-383).
+ * A checkpoint holds the run parameters, the caller's loop state, the device
+ * scalars carried from step to step (last NORMC, spin-2 Newton state) and the
+ * wave function as the device holds it (FP32 or FP64), followed by a 64-bit
+ * FNV-1a checksum; it is written to path.tmp and renamed.  Loading it into a
+ * handle created with the same parameters (bit-equal DT, DX, GAMMA, TAU,
+ * ALPHA, MAG; same grid, spin, precision and time mode, else FGPE_EINVAL)
+ * and advancing gives bit-identical results to a run never stopped.
+ * fgpe_checkpoint_info reads and verifies a file without a device. */
+typedef struct fgpe_loop_state {
+  int64_t ii;       /* steps done (II) */
+  double en_old;    /* EN of the previous STP event, for CONV_EN */
+  double user[6];   /* the caller's own values, stored and returned as is */
+} fgpe_loop_state;
+typedef struct fgpe_checkpoint_header {
+  int32_t version, dim, spin, precision, imag;
+  int32_t n[3];
+  double dt;
+  int64_t ii;
+  double en_old;
+  int64_t payload_bytes;
+} fgpe_checkpoint_header;
+int fgpe_checkpoint_save(fgpe_handle* h, const char* path, const fgpe_loop_state* st);
+int fgpe_checkpoint_load(fgpe_handle* h, const char* path, fgpe_loop_state* st);
+int fgpe_checkpoint_info(const char* path, fgpe_checkpoint_header* info);
+
+/* The field dumps of cgpe.x, formatted on `threads` host threads (0: all) and
+ * written in order; the bytes are those of the reference's formatted WRITE
+ * loops: records in the loop order i, j, k (k fastest), density |PHI|**2 and
+ * phase ATAN2(AIMAG, REAL) per component, Fw.d fields (exact value rounded to
+ * d decimals, ties to even; w asterisks on overflow), an empty record after
+ * each x where the reference writes one.  phi in the Fortran layout
+ * (interleaved complex double); x, y, z the grids the dump prints.
+ *   FGPE_DUMP_1D_TMP  (F11.5,10F14.6)                SRC/cgpe1d.f90:513-516
+ *   FGPE_DUMP_1D_SOL  (F11.5,10F18.10)               SRC/cgpe1d.f90:547-549
+ *   FGPE_DUMP_2D      (2F8.2,10F10.5) + empty record SRC/cgpe2d.f90:521-528,560-565
+ *   FGPE_DUMP_3D      (10F16.10)                     SRC/cgpe3d.f90:534-540,596-602
+ *   FGPE_DUMP_3D_XY   (2F10.4,10F14.8) at k = NZ/2+1 SRC/cgpe3d.f90:541-551
+ *   FGPE_DUMP_3D_XZ   (2F10.4,10F14.8) at j = NY/2+1 SRC/cgpe3d.f90:553-563
+ * fgpe_format_f writes one Fw.d field (w <= 40) and a NUL into out. */
+#define FGPE_DUMP_1D_TMP 1
+#define FGPE_DUMP_1D_SOL 2
+#define FGPE_DUMP_2D 3
+#define FGPE_DUMP_3D 4
+#define FGPE_DUMP_3D_XY 5
+#define FGPE_DUMP_3D_XZ 6
+int fgpe_write_dump(const char* path, int32_t kind, const int32_t* n, int32_t ms, const double* phi,
+                    const double* x, const double* y, const double* z, int32_t threads);
+int fgpe_format_f(double v, int32_t w, int32_t d, char* out, size_t cap);
+
+/* Reads ./INPUT/initial_sol_spinS.dat as cgpe.x's list-directed READ loop
+ * does (SRC/cgpe3d.f90:391-409, SRC/cgpe2d.f90:382-397, SRC/cgpe1d.f90:
+ * 377-388): one record per grid point in the order i, j, k (empty records
+ * skipped), x (1D) or x, y (2D) then density and phase per component;
+ * PHI = SQRT(density) EXP(i phase) into phi (Fortran layout).  x, y (may be
+ * null) receive the coordinates as the READ leaves them (the last record of
+ * each index).  Records are parsed on `threads` host threads (0: all).  Takes
+ * plain numbers separated by blanks or commas, D or E exponents; a record
+ * with fewer values, repeat counts (r*c), '/' or text is refused with
+ * FGPE_EINVAL. */
+int fgpe_read_initial_sol(const char* path, int32_t dim, const int32_t* n, int32_t ms, double* phi, double* x,
+                          double* y, int32_t threads);
+
 /* Device time in milliseconds of the last fgpe_run call, measured with HIP
  * events on the handle's stream (no reference counterpart; the reference
  * only times the whole program, SRC/cgpe3d.f90:218,606-614). */
