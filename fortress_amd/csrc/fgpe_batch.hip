@@ -180,6 +180,10 @@ struct fgpe_batch {
   size_t lds_max = 65536;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the length groups' independent step chains run on gstream[c] (group 0 on
+  // the batch's stream), forked from and joined back to it by gev
+  hipStream_t gstream[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t gev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<fgpe_params> p;
   std::vector<int> n, cls, sched;
   std::vector<long> off, poff;
@@ -368,7 +372,7 @@ template <typename K> void allow(K kernel, size_t bytes) {
 }
 
 // one x-pass launch over the cnt instances of length N at dlist + start
-template <typename T, int N, int Q> int xlaunch(fgpe_batch* b, int start, int cnt, int flags) {
+template <typename T, int N, int Q> int xlaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int flags) {
   constexpr int P = N / Q;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const int threads = P >= 256 ? P : 256, lpw = threads / P, nb = (N + lpw - 1) / lpw;
@@ -376,56 +380,56 @@ template <typename T, int N, int Q> int xlaunch(fgpe_batch* b, int start, int cn
   if (lds > b->lds_max) return fail(FGPE_EINVAL, "batch x pass needs %zu B of LDS", lds);
   auto kern = kg_xpass<T, N, Q>;
   allow(kern, lds);
-  hipLaunchKernelGGL(kern, dim3(cnt * nb), dim3(threads), lds, b->stream, (const GInst<T>*)b->ginst, b->dlist + start,
+  hipLaunchKernelGGL(kern, dim3(cnt * nb), dim3(threads), lds, st, (const GInst<T>*)b->ginst, b->dlist + start,
                      nb, flags);
   HIPCHK(hipGetLastError());
   if ((flags & XF_PHYS) && (flags & XF_NORM)) {
-    hipLaunchKernelGGL(kg_finalize<T>, dim3(cnt), dim3(1024), 0, b->stream, (const GInst<T>*)b->ginst,
+    hipLaunchKernelGGL(kg_finalize<T>, dim3(cnt), dim3(1024), 0, st, (const GInst<T>*)b->ginst,
                        b->dlist + start, nb);
     HIPCHK(hipGetLastError());
   }
   return 0;
 }
 // one k-pass launch (2D: y lines, KE, SOC) over the cnt instances
-template <typename T, int N, int Q> int klaunch(fgpe_batch* b, int start, int cnt) {
+template <typename T, int N, int Q> int klaunch(fgpe_batch* b, hipStream_t st, int start, int cnt) {
   constexpr int P = N / Q;
   const int nb = N / BTX;
   const size_t lds = kpass_lds_bytes<T, 1, N, Q, BTX>();
   if (lds > b->lds_max) return fail(FGPE_EINVAL, "batch k pass needs %zu B of LDS", lds);
   auto kern = kg_kpass<T, N, Q, BTX>;
   allow(kern, lds);
-  hipLaunchKernelGGL(kern, dim3(cnt * nb), dim3(BTX * P), lds, b->stream, (const GInst<T>*)b->ginst, b->dlist + start,
+  hipLaunchKernelGGL(kern, dim3(cnt * nb), dim3(BTX * P), lds, st, (const GInst<T>*)b->ginst, b->dlist + start,
                      nb, 1);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 // schedule dispatch (compile-time Q per schedule)
-template <typename T> int xpass_s(fgpe_batch* b, int s, int start, int cnt, int flags) {
+template <typename T> int xpass_s(fgpe_batch* b, hipStream_t st, int s, int start, int cnt, int flags) {
   constexpr bool F = sizeof(T) == 4;
   switch (s) {
-    case 0: return xlaunch<T, 64, F ? SQX32[0] : SQX64[0]>(b, start, cnt, flags);
-    case 1: return xlaunch<T, 64, F ? SQX32[1] : SQX64[1]>(b, start, cnt, flags);
-    case 2: return xlaunch<T, 128, F ? SQX32[2] : SQX64[2]>(b, start, cnt, flags);
-    case 3: return xlaunch<T, 128, F ? SQX32[3] : SQX64[3]>(b, start, cnt, flags);
-    case 4: return xlaunch<T, 192, F ? SQX32[4] : SQX64[4]>(b, start, cnt, flags);
-    case 5: return xlaunch<T, 192, F ? SQX32[5] : SQX64[5]>(b, start, cnt, flags);
-    case 6: return xlaunch<T, 256, F ? SQX32[6] : SQX64[6]>(b, start, cnt, flags);
-    case 7: return xlaunch<T, 256, F ? SQX32[7] : SQX64[7]>(b, start, cnt, flags);
+    case 0: return xlaunch<T, 64, F ? SQX32[0] : SQX64[0]>(b, st, start, cnt, flags);
+    case 1: return xlaunch<T, 64, F ? SQX32[1] : SQX64[1]>(b, st, start, cnt, flags);
+    case 2: return xlaunch<T, 128, F ? SQX32[2] : SQX64[2]>(b, st, start, cnt, flags);
+    case 3: return xlaunch<T, 128, F ? SQX32[3] : SQX64[3]>(b, st, start, cnt, flags);
+    case 4: return xlaunch<T, 192, F ? SQX32[4] : SQX64[4]>(b, st, start, cnt, flags);
+    case 5: return xlaunch<T, 192, F ? SQX32[5] : SQX64[5]>(b, st, start, cnt, flags);
+    case 6: return xlaunch<T, 256, F ? SQX32[6] : SQX64[6]>(b, st, start, cnt, flags);
+    case 7: return xlaunch<T, 256, F ? SQX32[7] : SQX64[7]>(b, st, start, cnt, flags);
   }
   return fail(FGPE_EINVAL, "bad schedule %d", s);
 }
-template <typename T> int kpass_s(fgpe_batch* b, int s, int start, int cnt) {
+template <typename T> int kpass_s(fgpe_batch* b, hipStream_t st, int s, int start, int cnt) {
   constexpr bool F = sizeof(T) == 4;
   switch (s) {
-    case 0: return klaunch<T, 64, F ? SQK32[0] : SQK64[0]>(b, start, cnt);
-    case 1: return klaunch<T, 64, F ? SQK32[1] : SQK64[1]>(b, start, cnt);
-    case 2: return klaunch<T, 128, F ? SQK32[2] : SQK64[2]>(b, start, cnt);
-    case 3: return klaunch<T, 128, F ? SQK32[3] : SQK64[3]>(b, start, cnt);
-    case 4: return klaunch<T, 192, F ? SQK32[4] : SQK64[4]>(b, start, cnt);
-    case 5: return klaunch<T, 192, F ? SQK32[5] : SQK64[5]>(b, start, cnt);
-    case 6: return klaunch<T, 256, F ? SQK32[6] : SQK64[6]>(b, start, cnt);
-    case 7: return klaunch<T, 256, F ? SQK32[7] : SQK64[7]>(b, start, cnt);
+    case 0: return klaunch<T, 64, F ? SQK32[0] : SQK64[0]>(b, st, start, cnt);
+    case 1: return klaunch<T, 64, F ? SQK32[1] : SQK64[1]>(b, st, start, cnt);
+    case 2: return klaunch<T, 128, F ? SQK32[2] : SQK64[2]>(b, st, start, cnt);
+    case 3: return klaunch<T, 128, F ? SQK32[3] : SQK64[3]>(b, st, start, cnt);
+    case 4: return klaunch<T, 192, F ? SQK32[4] : SQK64[4]>(b, st, start, cnt);
+    case 5: return klaunch<T, 192, F ? SQK32[5] : SQK64[5]>(b, st, start, cnt);
+    case 6: return klaunch<T, 256, F ? SQK32[6] : SQK64[6]>(b, st, start, cnt);
+    case 7: return klaunch<T, 256, F ? SQK32[7] : SQK64[7]>(b, st, start, cnt);
   }
   return fail(FGPE_EINVAL, "bad schedule %d", s);
 }
@@ -434,31 +438,39 @@ template <typename T> int kpass_s(fgpe_batch* b, int s, int start, int cnt) {
 int group_sched(const fgpe_batch* b, const Groups& g, int c) { return b->sched[g.ids[g.start[c]]]; }
 
 // nsteps steps of every listed instance, from real space back to real space:
-// the pass schedule of run_steps (fgpe_ops.hpp) for dim 2
+// the pass schedule of run_steps (fgpe_ops.hpp) for dim 2.  Length groups
+// share nothing, so each group's chain of launches runs on its own stream:
+// the small groups' launches fill the GPU beside the large ones'.
 template <typename T> int steps_t(fgpe_batch* b, const Groups& g, long nsteps) {
   if (nsteps <= 0 || g.total == 0) return 0;
   int rc;
   hipLaunchKernelGGL(kg_set_scale<T>, dim3(g.total), dim3(64), 0, b->stream, (const GInst<T>*)b->ginst, b->dlist);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->gev[0], b->stream));
+  for (int c = 1; c < NCLS; ++c) HIPCHK(hipStreamWaitEvent(b->gstream[c], b->gev[0], 0));
   const int imag = b->p[0].imag;
-  for (int c = 0; c < NCLS; ++c)
-    if (g.cnt[c] && (rc = xpass_s<T>(b, group_sched(b, g, c), g.start[c], g.cnt[c], XF_FWD))) return rc;
-  for (long s = 1; s <= nsteps; ++s) {
-    for (int c = 0; c < NCLS; ++c)
-      if (g.cnt[c] && (rc = kpass_s<T>(b, group_sched(b, g, c), g.start[c], g.cnt[c]))) return rc;
-    const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((imag || s == nsteps) ? XF_NORM : 0);
-    for (int c = 0; c < NCLS; ++c)
-      if (g.cnt[c] && (rc = xpass_s<T>(b, group_sched(b, g, c), g.start[c], g.cnt[c], flags))) return rc;
+  for (int c = 0; c < NCLS; ++c) {
+    if (!g.cnt[c]) continue;
+    hipStream_t st = b->gstream[c];
+    const int sc = group_sched(b, g, c);
+    if ((rc = xpass_s<T>(b, st, sc, g.start[c], g.cnt[c], XF_FWD))) return rc;
+    for (long s = 1; s <= nsteps; ++s) {
+      if ((rc = kpass_s<T>(b, st, sc, g.start[c], g.cnt[c]))) return rc;
+      const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((imag || s == nsteps) ? XF_NORM : 0);
+      if ((rc = xpass_s<T>(b, st, sc, g.start[c], g.cnt[c], flags))) return rc;
+    }
+    if (imag) {
+      const long np = (long)CLS_N[c] * CLS_N[c];
+      const int nb = (int)std::min<long>((np + 255) / 256, 64);
+      hipLaunchKernelGGL(kg_apply_scale<T>, dim3(g.cnt[c] * nb), dim3(256), 0, st, (const GInst<T>*)b->ginst,
+                         b->dlist + g.start[c], nb, np);
+      HIPCHK(hipGetLastError());
+    }
   }
-  if (imag)
-    for (int c = 0; c < NCLS; ++c)
-      if (g.cnt[c]) {
-        const long np = (long)CLS_N[c] * CLS_N[c];
-        const int nb = (int)std::min<long>((np + 255) / 256, 64);
-        hipLaunchKernelGGL(kg_apply_scale<T>, dim3(g.cnt[c] * nb), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
-                           b->dlist + g.start[c], nb, np);
-        HIPCHK(hipGetLastError());
-      }
+  for (int c = 1; c < NCLS; ++c) {
+    HIPCHK(hipEventRecord(b->gev[c], b->gstream[c]));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->gev[c], 0));
+  }
   return 0;
 }
 
@@ -638,6 +650,13 @@ int fgpe_batch_create(const fgpe_params* p, int32_t count, fgpe_batch** out) {
     return bail(fail(FGPE_EDEVICE, "hipStreamCreate failed"));
   if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
     return bail(fail(FGPE_EDEVICE, "hipEventCreate failed"));
+  b->gstream[0] = b->stream;
+  for (int c = 0; c < 4; ++c) {
+    // default priority, like the batch's own stream
+    if ((c > 0 && hipStreamCreateWithFlags(&b->gstream[c], hipStreamNonBlocking) != hipSuccess) ||
+        hipEventCreateWithFlags(&b->gev[c], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(FGPE_EDEVICE, "hipStreamCreate/hipEventCreate failed"));
+  }
   const size_t es = b->fp32 ? sizeof(float2) : sizeof(double2);
   long total = 0;
   b->n.resize(count);
@@ -680,6 +699,13 @@ void fgpe_batch_destroy(fgpe_batch* b) {
   for (void* a : b->allocs) (void)hipFree(a);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  for (int c = 0; c < 4; ++c) {
+    if (c > 0 && b->gstream[c]) {
+      (void)hipStreamSynchronize(b->gstream[c]);
+      (void)hipStreamDestroy(b->gstream[c]);
+    }
+    if (b->gev[c]) (void)hipEventDestroy(b->gev[c]);
+  }
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
