@@ -43,13 +43,14 @@ template <> struct Cfg<float> {
   static constexpr int QK = 8;   // 2D k pass and 3D pass D (all components)
   static constexpr int Q1 = 16;  // q1D whole-step kernel
   static constexpr int TX = 16;  // x columns per strided-line block (128 B rows)
-  // x columns per pass-C block: 256 threads, so three blocks share a CU and
-  // their load, transform and store phases overlap
-  static constexpr int TXK = 8;
+  // x columns per pass-D block (3D y lines, all components): 16 columns,
+  // whole 128 B rows, 512 threads (349 -> 321 us at 256^3 spin-2 against 8
+  // columns; 4 points per thread was slower)
+  static constexpr int TXK = 16;
   // q2D: a grid has only NY x lines, so pass A runs one line per block to
   // spread over the CUs, at 4 points per thread (radix-4 stages: half the
-  // roundings of radix 2); the k pass keeps QK and TXK (narrower tiles were
-  // slower in FP32)
+  // roundings of radix 2); the k pass runs 8 points per thread in tiles of 8
+  // columns (narrower tiles were slower in FP32)
   static constexpr int QX2 = 4;
   static constexpr int QK2 = 8;
   static constexpr int TXK2 = 8;
