@@ -62,7 +62,11 @@ template <> struct Cfg<double> {
   static constexpr int QK = 4;
   static constexpr int Q1 = 8;
   static constexpr int TX = 8;
-  static constexpr int TXK = 8;
+  // pass D in 16-column blocks (two 128 B rows).  8-column blocks of 96-point
+  // lines (128 threads, each line spread over both waves) gave results that
+  // changed from run to run on grids of 32x96x192 and larger; 16 columns are
+  // bit-reproducible on every grid tried
+  static constexpr int TXK = 16;
   // q2D: pass A at 2 points per thread, one line per block; the k pass in
   // tiles of 2 columns (256^2 spin-2: 13.0k -> 21.3k steps/s; 2 points per
   // thread in the k pass was slower)

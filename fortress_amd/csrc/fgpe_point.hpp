@@ -453,7 +453,17 @@ __global__ __launch_bounds__(OBS_TPB) void k_obs_line(const C_t<T>* __restrict__
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-  for (long b = blockIdx.x; b < ntiles; b += gridDim.x) {
+  // y and z tiles of L columns narrower than a 128 B line: the tiles that
+  // share a line go to blocks 8 apart (one XCD, dispatched together), so the
+  // line is fetched into one L2 once instead of once per tile
+  constexpr int SHARE = (AXIS != 0 && L * (int)sizeof(C) < 128) ? 128 / (L * (int)sizeof(C)) : 1;
+  const long full = ntiles / (8 * SHARE) * (8 * SHARE);
+  for (long bb = blockIdx.x; bb < ntiles; bb += gridDim.x) {
+    long b = bb;
+    if (SHARE > 1 && bb < full) {
+      const long r = bb % 8, q = (bb / 8) % SHARE;
+      b = bb - bb % (8 * SHARE) + r * SHARE + q;
+    }
     // line l of the tile: its (y, z) (x lines) or x (y, z lines) and validity
     long line0 = 0;
     int ox = 0, oy = 0, oz = 0;

@@ -46,8 +46,8 @@ def test_schedules_are_bit_identical(name):
             got = (s.download(), s.last_norms())
         print(name, "schedule", (zc, xc, ns), "in effect", eff, flush=True)
         assert eff[0] == (zc if zc and p.n[2] % zc == 0 else p.n[2])
-        # x-chunks hold whole pass-D block pairs: 32 columns in FP32, 16 in FP64
-        q = 32 if prec == FGPE_FP32 else 16
+        # x-chunks hold whole pass-D block pairs: 32 columns
+        q = 32
         assert eff[1] == (xc if xc and p.n[0] % xc == 0 and xc % q == 0 and xc < p.n[0] else p.n[0])
         assert eff[2] == ns
         if ref is None:
