@@ -478,6 +478,32 @@ def batch_solve(args, inps):
     print(json.dumps(line), flush=True)
 
 
+def gemm_bench(args):
+    """TFLOP/s of the surrogate network's matrix-core product (k_gemm) on the
+    products of the production 6-512x4-24576 network: inference of a batch of
+    1024 draws (forward of the output layer), its weight gradient, the input
+    gradient (split-K over 24576), the training step's batch-32 forward, and a
+    square 4096^3 product for scale."""
+    from fortress_amd import mlp
+    shapes = [("fwd_out_b1024", 1024, 24576, 512, 0, 0), ("dW_out_b1024", 512, 24576, 1024, 1, 0),
+              ("dH_out_b1024", 1024, 512, 24576, 0, 1), ("fwd_hidden_b1024", 1024, 512, 512, 0, 0),
+              ("fwd_out_b32", 32, 24576, 512, 0, 0), ("square_4096", 4096, 4096, 4096, 0, 0)]
+    res = {}
+    for name, m, n, k, ta, tb in shapes:
+        ms = mlp.gemm_time(m, n, k, ta, tb, reps=args.steps if args.steps < 200 else 50)
+        tf = 2.0 * m * n * k / (ms * 1e-3) / 1e12
+        res[name] = {"m": m, "n": n, "k": k, "ta": ta, "tb": tb, "ms": round(ms, 5), "TFLOPs": round(tf, 1)}
+        print(name, res[name], flush=True)
+    best = max(v["TFLOPs"] for v in res.values())
+    line = {"metric": "surrogate GEMM throughput (k_gemm, v_mfma_f32_32x32x16_bf16)", "value": best,
+            "unit": "TFLOP/s", "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 operands, fp32 accumulation",
+            "data": "synthetic operands filled on the device",
+            "roofline": {"bound": "mfma", "achieved": best, "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(best / 2500.0, 4)},
+            "shapes": res}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -520,6 +546,8 @@ def main():
     ap.add_argument("--config-niter", type=int, default=20000)
     ap.add_argument("--config-rt-steps", type=int, default=5000)
     ap.add_argument("--config-fp32-only", action="store_true", help="with --config2: also FP32 without the finish")
+    ap.add_argument("--gemm", action="store_true",
+                    help="instead of the 3D propagator, the surrogate network's GEMM TFLOP/s")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -535,6 +563,8 @@ def main():
         return config1_bench(args)
     if args.config2:
         return config2_bench(args)
+    if args.gemm:
+        return gemm_bench(args)
 
     from fortress_amd import fgpe
     from fortress_amd.abi import FGPE_FP32, FGPE_FP64

@@ -3,10 +3,11 @@
 //
 // Layers are y = act(x W + b) with W stored [K][N] (row major), FP32 master
 // weights and BF16 copies feeding the matrix cores.  Every product goes
-// through k_gemm: a 64x64 tile per 256-thread block, four waves each owning a
-// 32x32 quarter computed with v_mfma_f32_32x32x8bf16_1k (BF16 operands, FP32
-// accumulation), K staged through LDS in chunks of 32, optional transposed
-// operands and split-K with a deterministic second-stage sum.  The rest
+// through k_gemm: a 128x128 tile per 256-thread block, four waves each owning
+// a 64x64 quarter computed with gfx950's v_mfma_f32_32x32x16_bf16 (BF16
+// operands, FP32 accumulation), K staged through double-buffered LDS in chunks
+// of 32 with 16 B operand loads, optional transposed operands and split-K with
+// a deterministic second-stage sum.  The rest
 // (bias + ReLU with the BF16 copy, the MSE loss gradient, the ReLU mask, bias
 // gradients, Adam, minibatch gathering) are small elementwise kernels.
 // No reference counterpart: the reference starts every run from its Gaussian
@@ -38,8 +39,7 @@ using fgpe_impl::fail;
 namespace fgpe_mlp_dev {
 
 typedef uint16_t bf16;
-typedef short short4_t __attribute__((ext_vector_type(4)));
-typedef float float16_t __attribute__((ext_vector_type(16)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // round to nearest even
 __host__ __device__ __forceinline__ bf16 to_bf16(float f) {
@@ -50,62 +50,149 @@ __host__ __device__ __forceinline__ bf16 to_bf16(float f) {
 }
 __host__ __device__ __forceinline__ float from_bf16(bf16 h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
 
-constexpr int TM = 64, TN = 64, KC = 32;
+// Block tile BM x BN, K staged through LDS in chunks of BK (double-buffered).
+// LDS holds both operands k-contiguous per row (A by m, B by n), rows padded
+// to LDK elements (80 B) so a lane's 8 consecutive k are one 16 B read and
+// the 32 lanes of an MFMA operand hit different banks.
+constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+constexpr int PPR = BK / 8;                  // 8-element pieces per K-contiguous row
+constexpr int NPT = BM * BK / 8 / 256;       // pieces per thread per operand chunk
+static_assert(BM == 128 && BN == 128 && BK % 16 == 0 && NPT >= 1, "tile shape");
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// One 8-element piece of an operand tile: where it lies in global memory and
+// in LDS.  `row` indexes the LDS rows (m for A, n for B), `k` the K offset in
+// the chunk; `kmajor` pieces run along K in global memory (one 16 B LDS
+// store), the others along m/n (eight 2 B LDS stores at consecutive rows).
+struct Piece {
+  int row, k;
+};
+// Operand X with LDS row index r (m or n) and K index k: element address
+// X[k * ld + r] if the operand is stored K-major along r (transposed A,
+// plain B), else X[r * ld + k].
+template <bool KROWS>
+__device__ __forceinline__ void load_piece(const bf16* __restrict__ X, int ld, int r0, int rmax, int k0, int kend,
+                                           int v, bool vec, u32x4& out) {
+  if constexpr (!KROWS) {
+    // PPR pieces per row of BK: row r, k .. k+7 contiguous in memory
+    const int r = v / PPR, kq = (v % PPR) * 8;
+    const int gr = r0 + r, gk = k0 + kq;
+    if (vec && gr < rmax && gk + 8 <= kend) {
+      out = *reinterpret_cast<const u32x4*>(X + (long)gr * ld + gk);
+    } else {
+      unsigned short e[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) e[t] = (gr < rmax && gk + t < kend) ? X[(long)gr * ld + gk + t] : (unsigned short)0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) out[t] = (unsigned int)e[2 * t] | ((unsigned int)e[2 * t + 1] << 16);
+    }
+  } else {
+    // 16 pieces per K row of 128: k fixed, rows r .. r+7 contiguous in memory
+    const int k = v >> 4, rq = (v & 15) * 8;
+    const int gk = k0 + k, gr = r0 + rq;
+    if (vec && gk < kend && gr + 8 <= rmax) {
+      out = *reinterpret_cast<const u32x4*>(X + (long)gk * ld + gr);
+    } else {
+      unsigned short e[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) e[t] = (gk < kend && gr + t < rmax) ? X[(long)gk * ld + gr + t] : (unsigned short)0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) out[t] = (unsigned int)e[2 * t] | ((unsigned int)e[2 * t + 1] << 16);
+    }
+  }
+}
+template <bool KROWS> __device__ __forceinline__ void store_piece(bf16* __restrict__ S, int v, const u32x4& in) {
+  if constexpr (!KROWS) {
+    const int r = v / PPR, kq = (v % PPR) * 8;
+    *reinterpret_cast<u32x4*>(S + r * LDK + kq) = in;
+  } else {
+    const int k = v >> 4, rq = (v & 15) * 8;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) S[(rq + t) * LDK + k] = (bf16)((in[t >> 1] >> (16 * (t & 1))) & 0xffffu);
+  }
+}
 
 // C[M][N] (+)= op(A)[M][K] op(B)[K][N] over the K range of blockIdx.z.
 // TA: A stored [K][lda] (element (m,k) at k*lda+m), else [M][lda].
 // TB: B stored [N][ldb] (element (k,n) at n*ldb+k), else [K][ldb].
-// Split-K: block z writes its partial product to C + z*M*N.
+// vec: both operands 16 B aligned with leading dimensions a multiple of 8
+// (then full 8-element pieces are single loads).  Split-K: block z writes
+// its partial product to C + z*M*N.  256 threads: four waves, each a 64 x 64
+// quarter of the tile as 2 x 2 products of v_mfma_f32_32x32x16_bf16 (BF16
+// operands, FP32 accumulation).
 template <int TA, int TB>
 __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
-                                              float* __restrict__ C, int M, int N, int K, int kper) {
-  __shared__ bf16 As[TM][KC + 4];
-  __shared__ bf16 Bs[KC][TN + 4];
+                                              float* __restrict__ C, int M, int N, int K, int kper, int vec) {
+  __shared__ __align__(16) bf16 As[2][BM * LDK];
+  __shared__ __align__(16) bf16 Bs[2][BN * LDK];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int k_begin = blockIdx.z * kper, k_end = min(K, k_begin + kper);
   float* Cz = C + (long)blockIdx.z * M * N;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-  float16_t acc;
+  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  // A is K-rows when transposed, B when not
+  constexpr bool AK = TA != 0, BKR = TB == 0;
+  floatx16 acc[2][2];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-  for (int k0 = k_begin; k0 < k_end; k0 += KC) {
-    // stage A[m0..m0+63][k0..k0+31] and B[k0..k0+31][n0..n0+63], zero outside
-    for (int e = tid; e < TM * KC; e += 256) {
-      const int mm = TA ? (e % TM) : (e / KC), kk = TA ? (e / TM) : (e % KC);
-      const int m = m0 + mm, k = k0 + kk;
-      bf16 v = 0;
-      if (m < M && k < k_end) v = TA ? A[(long)k * lda + m] : A[(long)m * lda + k];
-      As[mm][kk] = v;
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  // BM * BK / 8 pieces of 8 per operand chunk: NPT per thread
+  u32x4 ra[NPT], rb[NPT];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+      load_piece<AK>(A, lda, m0, M, k0, k_end, tid + 256 * p, vec, ra[p]);
+      load_piece<BKR>(B, ldb, n0, N, k0, k_end, tid + 256 * p, vec, rb[p]);
     }
-    for (int e = tid; e < KC * TN; e += 256) {
-      const int kk = TB ? (e % KC) : (e / TN), nn = TB ? (e / KC) : (e % TN);
-      const int k = k0 + kk, n = n0 + nn;
-      bf16 v = 0;
-      if (n < N && k < k_end) v = TB ? B[(long)n * ldb + k] : B[(long)k * ldb + n];
-      Bs[kk][nn] = v;
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+      store_piece<AK>(As[buf], tid + 256 * p, ra[p]);
+      store_piece<BKR>(Bs[buf], tid + 256 * p, rb[p]);
     }
-    __syncthreads();
+  };
+  fetch(k_begin);
+  stash(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
+    const bool more = k0 + BK < k_end;
+    if (more) fetch(k0 + BK);  // in flight during this chunk's products
 #pragma unroll
-    for (int ks = 0; ks < KC; ks += 8) {
-      // lane l: A rows wr + l%32, k = ks + 4(l/32) .. +3; B cols wc + l%32, same k
-      const int kb = ks + 4 * (lane >> 5);
-      short4_t a, b;
+    for (int ks = 0; ks < BK; ks += 16) {
+      // lane l: row (m or n) l%32 of its 32-row block, k = ks + 8(l/32) .. +7
+      const int kk = ks + 8 * (lane >> 5);
+      bf16x8 a[2], b[2];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        a[t] = (short)As[wr + (lane & 31)][kb + t];
-        b[t] = (short)Bs[kb + t][wc + (lane & 31)];
+      for (int i = 0; i < 2; ++i) {
+        a[i] = *reinterpret_cast<const bf16x8*>(&As[buf][(wm + 32 * i + (lane & 31)) * LDK + kk]);
+        b[i] = *reinterpret_cast<const bf16x8*>(&Bs[buf][(wn + 32 * i + (lane & 31)) * LDK + kk]);
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (more) stash(buf ^ 1);  // the other buffer's readers passed the last barrier
     __syncthreads();
+    buf ^= 1;
   }
   // D layout: register r holds row 8(r/4) + 4(l/32) + r%4, column l%32
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int m = m0 + wr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), n = n0 + wc + (lane & 31);
-    if (m < M && n < N) Cz[(long)m * N + n] = acc[r];
-  }
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int n = n0 + wn + 32 * j + (lane & 31);
+        if (m < M && n < N) Cz[(long)m * N + n] = acc[i][j][r];
+      }
 }
 
 // out[i] = sum_z part[z*n + i] (fixed order)
@@ -197,6 +284,17 @@ __global__ void k_gather(const float* __restrict__ src, int width, const int* __
   }
 }
 
+// x[i] = a BF16 value in [-1, 1) from a hash of i
+__global__ void k_fill_bf16(bf16* __restrict__ x, long n, unsigned int seed) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    unsigned int h = (unsigned int)i * 2654435761u ^ seed;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    x[i] = to_bf16((float)(h >> 8) * (1.0f / 8388608.0f) - 1.0f);
+  }
+}
+
 }  // namespace fgpe_mlp_dev
 
 using namespace fgpe_mlp_dev;
@@ -238,17 +336,19 @@ int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 4096); }
 
 // C[M][N] = op(A) op(B), split over K when the output has few tiles
 int gemm(fgpe_mlp* m, int ta, int tb, const bf16* A, int lda, const bf16* B, int ldb, float* C, int M, int N, int K) {
-  const int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+  const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   int splits = 1;
-  while (tiles * splits < 256 && K / (splits * 2) >= 4 * KC) splits *= 2;
+  while (tiles * splits < 256 && K / (splits * 2) >= 4 * BK) splits *= 2;
   const size_t need = (size_t)splits * M * N;
   if (splits > 1 && need > m->part_elems) splits = 1;
   int kper = (K + splits - 1) / splits;
-  kper = (kper + KC - 1) / KC * KC;
+  kper = (kper + BK - 1) / BK * BK;
   splits = (K + kper - 1) / kper;
   float* out = splits > 1 ? m->part : C;
-  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
-#define GEMM(a, b) hipLaunchKernelGGL((k_gemm<a, b>), grid, dim3(256), 0, m->stream, A, lda, B, ldb, out, M, N, K, kper)
+  const int vec = ((uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0 && lda % 8 == 0 && ldb % 8 == 0) ? 1 : 0;
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, splits);
+#define GEMM(a, b) \
+  hipLaunchKernelGGL((k_gemm<a, b>), grid, dim3(256), 0, m->stream, A, lda, B, ldb, out, M, N, K, kper, vec)
   if (!ta && !tb)
     GEMM(0, 0);
   else if (ta && !tb)
@@ -563,3 +663,44 @@ int fgpe_mlp_gemm_test(int32_t ta, int32_t tb, const uint16_t* A, const uint16_t
 }
 
 }  // extern "C"
+
+int fgpe_mlp_gemm_time(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps, double* ms) {
+  if (!ms || M < 1 || N < 1 || K < 1 || reps < 1) return fail(FGPE_EINVAL, "bad GEMM timing argument");
+  fgpe_mlp tmp;
+  if (hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(FGPE_EDEVICE, "hipStreamCreate failed");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto done = [&](int rc) {
+    (void)hipStreamSynchronize(tmp.stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void* p : tmp.allocs) (void)hipFree(p);
+    (void)hipStreamDestroy(tmp.stream);
+    return rc;
+  };
+  bf16 *dA, *dB;
+  float* dC;
+  tmp.part_elems = (size_t)M * N * 16;
+  int rc;
+  if ((rc = malloc_dev(&tmp, (void**)&dA, (size_t)M * K * sizeof(bf16))) ||
+      (rc = malloc_dev(&tmp, (void**)&dB, (size_t)K * N * sizeof(bf16))) ||
+      (rc = malloc_dev(&tmp, (void**)&dC, (size_t)M * N * sizeof(float))) ||
+      (rc = malloc_dev(&tmp, (void**)&tmp.part, tmp.part_elems * sizeof(float))))
+    return done(rc);
+  hipLaunchKernelGGL(k_fill_bf16, dim3(grid1((long)M * K)), dim3(256), 0, tmp.stream, dA, (long)M * K, 1u);
+  hipLaunchKernelGGL(k_fill_bf16, dim3(grid1((long)K * N)), dim3(256), 0, tmp.stream, dB, (long)K * N, 2u);
+  if (hipGetLastError() != hipSuccess) return done(fail(FGPE_EDEVICE, "fill failed"));
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return done(fail(FGPE_EDEVICE, "hipEventCreate failed"));
+  const int lda = ta ? M : K, ldb = tb ? K : N;
+  if ((rc = gemm(&tmp, ta, tb, dA, lda, dB, ldb, dC, M, N, K))) return done(rc);
+  if (hipEventRecord(e0, tmp.stream) != hipSuccess) return done(fail(FGPE_EDEVICE, "hipEventRecord failed"));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = gemm(&tmp, ta, tb, dA, lda, dB, ldb, dC, M, N, K))) return done(rc);
+  float t = 0.0f;
+  if (hipEventRecord(e1, tmp.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+      hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+    return done(fail(FGPE_EDEVICE, "GEMM timing failed"));
+  *ms = (double)t / reps;
+  return done(0);
+}

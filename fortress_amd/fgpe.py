@@ -68,6 +68,7 @@ EXPORTS = (
     "fgpe_mlp_set_dataset",
     "fgpe_mlp_train_step",
     "fgpe_mlp_gemm_test",
+    "fgpe_mlp_gemm_time",
 )
 
 BATCH_NSCHED = 8

@@ -30,9 +30,10 @@ def _bind():
     L.fgpe_mlp_set_dataset.argtypes = [vp, fp, fp, i32]
     L.fgpe_mlp_train_step.argtypes = [vp, ip, i32, C.c_float, dp]
     L.fgpe_mlp_gemm_test.argtypes = [i32, i32, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), fp, i32, i32, i32]
+    L.fgpe_mlp_gemm_time.argtypes = [i32, i32, i32, i32, i32, i32, C.POINTER(C.c_double)]
     for n in ("fgpe_mlp_create", "fgpe_mlp_param_count", "fgpe_mlp_get_params", "fgpe_mlp_set_params",
               "fgpe_mlp_forward", "fgpe_mlp_gradients", "fgpe_mlp_set_dataset", "fgpe_mlp_train_step",
-              "fgpe_mlp_gemm_test"):
+              "fgpe_mlp_gemm_test", "fgpe_mlp_gemm_time"):
         getattr(L, n).restype = C.c_int
     _bound = True
     return L
@@ -62,6 +63,16 @@ def gemm(a_bits, b_bits, m, n, k, ta=False, tb=False):
     _check(L.fgpe_mlp_gemm_test(int(ta), int(tb), a.ctypes.data_as(C.POINTER(C.c_uint16)),
                                 b.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(c), m, n, k), "fgpe_mlp_gemm_test")
     return c
+
+
+def gemm_time(m, n, k, ta=False, tb=False, reps=20):
+    """Milliseconds per product C[m][n] = op(A) op(B) on device-resident BF16
+    operands (the mean of reps launches after one untimed launch)."""
+    L = _bind()
+    ms = C.c_double()
+    _check(L.fgpe_mlp_gemm_time(int(ta), int(tb), int(m), int(n), int(k), int(reps), C.byref(ms)),
+           "fgpe_mlp_gemm_time")
+    return ms.value
 
 
 class MLP:

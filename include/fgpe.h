@@ -384,6 +384,10 @@ int fgpe_mlp_train_step(fgpe_mlp* m, const int32_t* idx, int32_t batch, float lr
  * (ta: A given as [K][M]; tb: B given as [N][K]). */
 int fgpe_mlp_gemm_test(int32_t ta, int32_t tb, const uint16_t* a, const uint16_t* b, float* c, int32_t m, int32_t n,
                        int32_t k);
+/* Device time of that product on resident operands (filled on the device),
+ * averaged over reps launches after one untimed launch, in *ms; 2 m n k / ms
+ * is its FLOP rate. */
+int fgpe_mlp_gemm_time(int32_t ta, int32_t tb, int32_t m, int32_t n, int32_t k, int32_t reps, double* ms);
 
 #ifdef __cplusplus
 }
