@@ -76,6 +76,27 @@ def cpu_baseline(phi0, n, steps):
                       "(oracle/fgpe_oracle.c, OpenMP, %d threads), %.1f s" % (steps, n, threads, dt)}
 
 
+def reference_cost(n, gpu_steps_per_s):
+    """The reference's own step cannot be timed (it does not build here), so
+    oracle/refcost times the two library calls that dominate it -- the FFTW
+    3D transforms of FFT/BFT and the two ZHEEV calls per point of the spin-2
+    SE step -- on the same host threads; their sum bounds the reference's
+    step time from below."""
+    import subprocess
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    exe = os.path.join(ROOT, "oracle", "refcost")
+    try:
+        out = subprocess.run([exe, str(n), str(threads), "200000"], capture_output=True, text=True, timeout=300)
+        if out.returncode != 0:
+            return {"error": (out.stderr or out.stdout).strip()[-300:]}
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+    except (OSError, subprocess.SubprocessError, ValueError) as e:
+        return {"error": str(e)}
+    r["reference_steps_per_s_at_most"] = round(1.0 / r["step_lower_bound_s"], 5)
+    r["gpu_vs_reference_at_least"] = round(gpu_steps_per_s * r["step_lower_bound_s"], 1)
+    return r
+
+
 def accuracy(args):
     """FP32 against FP64 propagation of the same workload from the same state:
     relative L2 of the component densities |phi_m|^2 at checkpoints."""
@@ -513,6 +534,8 @@ def main():
     ap.add_argument("--prep-steps", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-refcost", action="store_true",
+                    help="skip timing the reference's FFTW and ZHEEV calls beside the CPU baseline")
     ap.add_argument("--schedule", default=None, metavar="ZCHUNK,XCHUNK,STREAMS",
                     help="fixed 3D schedule instead of the autotuner (0,0,1: whole-grid launches, as profiled)")
     ap.add_argument("--tune-steps", type=int, default=20)
@@ -692,6 +715,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(phi0, args.n, args.cpu_steps)
+        if not args.no_refcost:
+            line["cpu_baseline"]["reference_cost_model"] = reference_cost(args.n, steps_per_s)
     print(json.dumps(line), flush=True)
 
 
