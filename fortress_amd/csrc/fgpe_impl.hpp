@@ -90,6 +90,13 @@ __host__ __device__ constexpr int q_for(int n, int qdef) { return (n & (n - 1)) 
 template <typename T> constexpr int x_pass_q(int dim, int nx) {
   return q_for(nx, dim == 2 ? Cfg<T>::QX2 : Cfg<T>::QX);
 }
+// Points per thread of 3D pass C (z lines): the default, but at most 8 for
+// lines of 512 points and more, whose 16-column blocks need 70 KB of LDS (two
+// blocks per CU): 8 points per thread gives each block twice the waves in the
+// same LDS (512^3 FP32 pass C: 4.20 -> 2.89 ms).
+template <typename T> constexpr int z_pass_q(int nz) {
+  return q_for(nz, (nz >= 512 && Cfg<T>::QZ > 8) ? 8 : Cfg<T>::QZ);
+}
 // Blocks of one pass-A launch: P = NX/Q threads per x line, lines packed into
 // blocks of at least 256 threads (3D) or of one wave (q2D, where the grid has
 // few lines).  Each block writes MS NORMC partials.

@@ -94,7 +94,7 @@ template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
 
 // 3D pass C (z lines, KE) over x in [x0, x0 + xw) on stream st.
 template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw) {
-  constexpr int Q = q_for(N, Cfg<T>::QZ), P = N / Q, TX = Cfg<T>::TX;
+  constexpr int Q = z_pass_q<T>(N), P = N / Q, TX = Cfg<T>::TX;
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "z pass needs %zu B of LDS", lds);
