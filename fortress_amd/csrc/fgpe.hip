@@ -88,7 +88,7 @@ template <typename T> int build_tables(fgpe_handle* h) {
   // in 3D and by the k-space pass in 2D, y again by the all-component pass D
   // in 3D (its own table, QK points per thread), z by pass C
   const int qx = h->dim == 1 ? q_for(h->nx, Cfg<T>::Q1) : x_pass_q<T>(h->dim, h->nx);
-  const int qy = q_for(h->ny, h->dim == 3 ? Cfg<T>::QY : Cfg<T>::QK2);
+  const int qy = h->dim == 3 ? y_pass_q<T>(h->ny) : q_for(h->ny, Cfg<T>::QK2);
   const int qz = z_pass_q<T>(h->nz);
   auto kx = kgrid(h->nx, p.d[0]), ky = kgrid(h->ny, p.d[1]), kz = kgrid(h->nz, p.d[2]);
   auto xx = xgrid(h->nx, p.d[0]), yy = xgrid(h->ny, p.d[1]), zz = xgrid(h->nz, p.d[2]);

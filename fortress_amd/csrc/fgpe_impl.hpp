@@ -62,9 +62,10 @@ template <> struct Cfg<double> {
   static constexpr int QK = 4;
   static constexpr int Q1 = 8;
   static constexpr int TX = 8;
-  // pass D in 16-column blocks (two 128 B rows).  8-column blocks of 96-point
-  // lines (128 threads, each line spread over both waves) gave results that
-  // changed from run to run on grids of 32x96x192 and larger; 16 columns are
+  // pass D in 16-column blocks (two 128 B rows; 8 for 512-point lines, where
+  // 16 columns would exceed 1024 threads).  8-column blocks of 96-point lines
+  // (128 threads, each line spread over both waves) gave results that changed
+  // from run to run on grids of 32x96x192 and larger; 16 columns are
   // bit-reproducible on every grid tried
   static constexpr int TXK = 16;
   // q2D: pass A at 2 points per thread, one line per block; the k pass in
@@ -90,13 +91,13 @@ __host__ __device__ constexpr int q_for(int n, int qdef) { return (n & (n - 1)) 
 template <typename T> constexpr int x_pass_q(int dim, int nx) {
   return q_for(nx, dim == 2 ? Cfg<T>::QX2 : Cfg<T>::QX);
 }
-// Points per thread of 3D pass C (z lines): the default, but at most 8 for
-// lines of 512 points and more, whose 16-column blocks need 70 KB of LDS (two
-// blocks per CU): 8 points per thread gives each block twice the waves in the
-// same LDS (512^3 FP32 pass C: 4.20 -> 2.89 ms).
-template <typename T> constexpr int z_pass_q(int nz) {
-  return q_for(nz, (nz >= 512 && Cfg<T>::QZ > 8) ? 8 : Cfg<T>::QZ);
-}
+// Points per thread of 3D passes B (y lines) and C (z lines): the default,
+// but at most 8 for lines of 512 points and more, whose 16-column blocks need
+// 70 KB of LDS (two blocks per CU): 8 points per thread gives each block twice
+// the waves in the same LDS (512^3 FP32 pass C: 4.20 -> 2.89 ms).
+template <typename T> constexpr int long_line_q(int n, int qdef) { return q_for(n, (n >= 512 && qdef > 8) ? 8 : qdef); }
+template <typename T> constexpr int y_pass_q(int ny) { return long_line_q<T>(ny, Cfg<T>::QY); }
+template <typename T> constexpr int z_pass_q(int nz) { return long_line_q<T>(nz, Cfg<T>::QZ); }
 // Blocks of one pass-A launch: P = NX/Q threads per x line, lines packed into
 // blocks of at least 256 threads (3D) or of one wave (q2D, where the grid has
 // few lines).  Each block writes MS NORMC partials.

@@ -63,7 +63,8 @@ template <typename T, int SPIN, int N> int launch_fin(fgpe_handle* h) {
 // [z0, z0 + zc) on stream st.
 template <typename T, int N>
 int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc) {
-  constexpr int Q = q_for(N, Cfg<T>::QY), P = N / Q, TX = Cfg<T>::TX;
+  constexpr int Q = y_pass_q<T>(N), P = N / Q, TX = Cfg<T>::TX;
+  static_assert(TX * P <= 1024, "pass B block shape");
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "y pass needs %zu B of LDS", lds);
@@ -80,6 +81,7 @@ int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw,
 // 2D k pass (y lines, KE, SOC), all of the grid.
 template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
   constexpr int Q = q_for(N, Cfg<T>::QK2), P = N / Q, TX = Cfg<T>::TXK2;
+  static_assert(TX * P <= 1024, "2D k pass block shape");
   const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
   auto kern = k_kpass<T, SPIN, N, Q, TX>;
@@ -95,6 +97,7 @@ template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
 // 3D pass C (z lines, KE) over x in [x0, x0 + xw) on stream st.
 template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw) {
   constexpr int Q = z_pass_q<T>(N), P = N / Q, TX = Cfg<T>::TX;
+  static_assert(TX * P <= 1024, "pass C block shape");
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   const size_t lds = TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "z pass needs %zu B of LDS", lds);
@@ -110,7 +113,10 @@ template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t
 
 // 3D pass D (SOC, inverse y, 1/N) over x in [x0, x0 + xw) on stream st.
 template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
-  constexpr int Q = q_for(N, Cfg<T>::QK), P = N / Q, TX = Cfg<T>::TXK;
+  // columns per block: TXK, fewer where P = N/Q threads per line would make
+  // the block exceed 1024 threads (FP64 512-point lines: 8)
+  constexpr int Q = q_for(N, Cfg<T>::QK), P = N / Q, TX = Cfg<T>::TXK * P <= 1024 ? Cfg<T>::TXK : 1024 / P;
+  static_assert(TX * P <= 1024 && TX >= 8, "pass D block shape");
   const size_t lds = dpass_lds_bytes<T, SPIN, N, Q, TX>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "d pass needs %zu B of LDS", lds);
   auto kern = k_dpass<T, SPIN, N, Q, TX>;

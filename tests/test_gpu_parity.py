@@ -71,8 +71,10 @@ CASES = {
                                      natoms=100000, nu=(20, 20, 20)), 10),
     "3d_s1_soc_imag_32x96x192": (case(3, 1, (32, 96, 192), (0.8, 0.3, 0.15), 5e-4, (0.5, 0.5, 0.5),
                                       natoms=100000, nu=(20, 20, 20)), 10),
-    # 512-point z lines (pass C at 8 points per thread in FP32)
+    # 512-point y and z lines (passes B and C at 8 points per thread in FP32)
     "3d_s2_soc_real_32x32x512": (case(3, 2, (32, 32, 512), (0.4, 0.4, 0.05), 5e-4, (0.5, 0.3, 0.7), imag=0,
+                                      natoms=100000, nu=(20, 20, 20)), 10),
+    "3d_s1_soc_imag_32x512x32": (case(3, 1, (32, 512, 32), (0.4, 0.05, 0.4), 5e-4, (0.6, 0.4, 0.5),
                                       natoms=100000, nu=(20, 20, 20)), 10),
 }
 
