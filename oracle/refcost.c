@@ -74,7 +74,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   const int n = atoi(argv[1]), threads = atoi(argv[2]);
-  const long sample = atol(argv[3]);
+  long sample = atol(argv[3]);
   const int ms = 5;
   if (n < 8 || threads < 1 || sample < threads) {
     fprintf(stderr, "refcost: bad arguments\n");
@@ -100,6 +100,7 @@ int main(int argc, char** argv) {
 
   /* ---- FFT + BFT of one step: MS components each way */
   const long npts = (long)n * n * n;
+  if (sample > npts) sample = npts; /* each sampled point updates its own entries */
   double complex* phi = malloc(sizeof(double complex) * npts * ms);
   double complex* phif = malloc(sizeof(double complex) * npts * ms);
   fftw_complex_t* wf = malloc(sizeof(fftw_complex_t) * npts);
@@ -183,7 +184,7 @@ int main(int argc, char** argv) {
       const double cdt = 5e-4;
 #pragma omp for schedule(static)
       for (long q = 0; q < sample; ++q) {
-        double complex* p = phi + (q % npts) * 1; /* any five values of the state */
+        double complex* p = phi + q; /* the point's five components */
         for (int i = 0; i < 25; ++i) a[i] = 0.0;
         const double complex c12 = urand(&s) + I * urand(&s), c13 = urand(&s) + I * urand(&s),
                              c23 = urand(&s) + I * urand(&s), c34 = urand(&s) + I * urand(&s),
