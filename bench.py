@@ -300,31 +300,47 @@ def surrogate_bench(args):
     rng = np.random.default_rng(args.seed)
     train = [S.draw(rng) for _ in range(args.train)]
     held = [S.draw(rng) for _ in range(args.held_out)]
-    t0 = time.perf_counter()
+    # a slice of the held-out draws (the evaluation of a large draw set split
+    # over several runs of the same trained network)
+    held = held[args.held_offset:args.held_offset + (args.held_count or len(held))]
     def log(tag):
         return lambda k, n: print(json.dumps({tag: {"steps": k, "converged": n}}), flush=True)
-    feats, targets, tsteps = S.make_dataset(train, tol=args.train_tol, max_steps=args.max_steps,
-                                            log=log("training_states"))
-    gen_s = time.perf_counter() - t0
-    print(json.dumps({"train_states": len(train), "generation_s": round(gen_s, 1),
-                      "train_steps_median": int(np.median(tsteps))}), flush=True)
-    # the last tenth of the training draws is kept out of training as a
-    # validation set (same distribution as the held-out draws)
-    nv = max(len(train) // 10, 1)
     sur = S.Surrogate(hidden=args.hidden, layers=4, max_batch=64, seed=args.seed)
-    t1 = time.perf_counter()
-    hist = sur.fit(feats[:-nv], targets[:-nv], epochs=args.epochs, batch=min(32, len(train) - nv), lr=args.lr,
-                   lr_decay=args.lr_decay)
-    train_s = time.perf_counter() - t1
-    pred = sur.predict_vectors(feats[-nv:])
-    tv = targets[-nv:]
-    # validation error relative to the targets' spread (1 = no better than
-    # the mean state), and the spread of the predictions across the
-    # validation draws against that of the targets (0: the inputs are ignored)
-    val = float(np.mean((pred - tv) ** 2) / np.mean((tv - tv.mean(axis=0)) ** 2))
-    spread = float(np.mean(pred.std(axis=0)) / np.mean(tv.std(axis=0)))
-    print(json.dumps({"train_loss_first": hist[0], "train_loss_last": hist[-1], "validation_loss": val,
-                      "prediction_spread": spread, "train_s": round(train_s, 1)}), flush=True)
+    hist, val, spread, gen_s, train_s = [float("nan")], None, None, None, None
+    if args.surrogate_load:
+        sur.load(args.surrogate_load)
+    else:
+        t0 = time.perf_counter()
+        feats, targets, tsteps = S.make_dataset(train, tol=args.train_tol, max_steps=args.max_steps,
+                                                log=log("training_states"))
+        gen_s = time.perf_counter() - t0
+        print(json.dumps({"train_states": len(train), "generation_s": round(gen_s, 1),
+                          "train_steps_median": int(np.median(tsteps))}), flush=True)
+        # the last tenth of the training draws is kept out of training as a
+        # validation set (same distribution as the held-out draws)
+        nv = max(len(train) // 10, 1)
+        t1 = time.perf_counter()
+        def epoch_log(e, loss):
+            if e % 25 == 0 or e == args.epochs - 1:
+                print(json.dumps({"epoch": e, "loss": loss, "s": round(time.perf_counter() - t1, 1)}), flush=True)
+        hist = sur.fit(feats[:-nv], targets[:-nv], epochs=args.epochs, batch=min(32, len(train) - nv), lr=args.lr,
+                       lr_decay=args.lr_decay, log=epoch_log)
+        train_s = time.perf_counter() - t1
+        pred = sur.predict_vectors(feats[-nv:])
+        tv = targets[-nv:]
+        # validation error relative to the targets' spread (1 = no better than
+        # the mean state), and the spread of the predictions across the
+        # validation draws against that of the targets (0: the inputs are ignored)
+        val = float(np.mean((pred - tv) ** 2) / np.mean((tv - tv.mean(axis=0)) ** 2))
+        spread = float(np.mean(pred.std(axis=0)) / np.mean(tv.std(axis=0)))
+        print(json.dumps({"train_loss_first": hist[0], "train_loss_last": hist[-1], "validation_loss": val,
+                          "prediction_spread": spread, "train_s": round(train_s, 1)}), flush=True)
+        del feats, targets
+        if args.surrogate_save:
+            sur.save(args.surrogate_save)
+        if args.surrogate_train_only:
+            sur.close()
+            return
     params = [S.problem(d, args.n) for d in held]
     with fgpe.Batch(params) as b:
         b.initial_guess()
@@ -371,9 +387,11 @@ def surrogate_bench(args):
                    "network": "6-%dx4-24576 (3 complex components on 64^2), BF16 MFMA, FP32 accumulation"
                               % args.hidden,
                    "train_states": len(train), "train_tol": args.train_tol, "held_out": len(held),
+                   "held_offset": args.held_offset, "network_loaded": bool(args.surrogate_load),
                    "epochs": args.epochs, "lr": args.lr, "lr_decay": args.lr_decay, "max_steps": args.max_steps},
         "train_loss_last": hist[-1], "validation_loss": val, "prediction_spread": spread,
-        "data_generation_s": round(gen_s, 1), "training_s": round(train_s, 1),
+        "data_generation_s": None if gen_s is None else round(gen_s, 1),
+        "training_s": None if train_s is None else round(train_s, 1),
         "dtype": "fp64 solves, bf16/fp32 network",
     }
     print(json.dumps(out), flush=True)
@@ -547,6 +565,13 @@ def main():
                     help="instead of the propagator throughput, the learned initial guess's time-to-TOL gain")
     ap.add_argument("--train", type=int, default=4000)
     ap.add_argument("--held-out", type=int, default=128)
+    ap.add_argument("--held-offset", type=int, default=0, help="with --surrogate: first held-out draw evaluated")
+    ap.add_argument("--held-count", type=int, default=0, help="with --surrogate: held-out draws evaluated (0: all)")
+    ap.add_argument("--surrogate-save", default=None, help="with --surrogate: save the trained network (.npz)")
+    ap.add_argument("--surrogate-load", default=None,
+                    help="with --surrogate: evaluate a saved network instead of generating data and training")
+    ap.add_argument("--surrogate-train-only", action="store_true",
+                    help="with --surrogate: stop after training (and saving)")
     ap.add_argument("--train-tol", type=float, default=1e-6)
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--epochs", type=int, default=1200)

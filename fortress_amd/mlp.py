@@ -161,9 +161,9 @@ class MLP:
                                          C.byref(loss)), "fgpe_mlp_train_step")
         return loss.value
 
-    def train(self, epochs, batch, lr, seed=0, lr_decay=1.0):
+    def train(self, epochs, batch, lr, seed=0, lr_decay=1.0, log=None):
         """Shuffled minibatch epochs over the device dataset; returns the mean
-        loss of each epoch."""
+        loss of each epoch (log(epoch, loss) is called after each)."""
         rng = np.random.default_rng(seed)
         hist = []
         for e in range(epochs):
@@ -173,5 +173,7 @@ class MLP:
                 tot += self.train_step(perm[s:s + batch], lr)
                 nb += 1
             hist.append(tot / max(nb, 1))
+            if log:
+                log(e, hist[-1])
             lr *= lr_decay
         return hist

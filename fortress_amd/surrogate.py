@@ -124,7 +124,7 @@ class Surrogate:
         self.net = MLP(dims, max_batch=max_batch, seed=seed)
         self.mean, self.sd = 0.0, 1.0
 
-    def fit(self, feats, targets, epochs, batch, lr, seed=0, lr_decay=0.97):
+    def fit(self, feats, targets, epochs, batch, lr, seed=0, lr_decay=0.97, log=None):
         """Trains on standardised targets: per output, minus its mean over the
         training set, over its spread (floored at 1e-3 of the largest spread),
         so the loss weighs how the state changes with the parameters rather
@@ -134,7 +134,7 @@ class Surrogate:
         sd = t.std(axis=0)
         self.sd = np.maximum(sd, 1e-3 * sd.max()) if sd.max() > 0 else np.ones_like(sd)
         self.net.set_dataset(feats, ((t - self.mean) / self.sd).astype(np.float32))
-        return self.net.train(epochs, batch, lr, seed=seed, lr_decay=lr_decay)
+        return self.net.train(epochs, batch, lr, seed=seed, lr_decay=lr_decay, log=log)
 
     def predict_vectors(self, feats):
         return self.net.forward(feats) * self.sd + self.mean
@@ -144,6 +144,20 @@ class Surrogate:
 
     def guess(self, d, n):
         return upsample(self.predict(d), n)
+
+    def save(self, path):
+        """The trained state (FP32 master weights, the targets' mean and
+        spread) as an .npz file that load() restores into a network of the
+        same shape."""
+        np.savez(path, params=self.net.get_params(), mean=np.asarray(self.mean, np.float64),
+                 sd=np.asarray(self.sd, np.float64), dims=np.asarray(self.net.dims, np.int64))
+
+    def load(self, path):
+        z = np.load(path, allow_pickle=False)
+        if list(z["dims"]) != list(self.net.dims):
+            raise ValueError("saved network %s does not match %s" % (list(z["dims"]), self.net.dims))
+        self.net.set_params(z["params"])
+        self.mean, self.sd = z["mean"], z["sd"]
 
     def close(self):
         self.net.close()

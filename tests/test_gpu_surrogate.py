@@ -62,3 +62,28 @@ def test_surrogate_guess_is_closer_and_converges_sooner():
     sur.close()
     assert np.mean(d_sur) < np.mean(d_plain)
     assert sum(steps_sur) < sum(steps_plain)
+
+
+def test_saved_network_predicts_the_same(tmp_path):
+    """save() and load() carry the trained state between processes: a fresh
+    network loaded from the file predicts bit for bit what the saved one
+    does, and a network of another shape is refused."""
+    rng = np.random.default_rng(3)
+    draws = [S.draw(rng) for _ in range(4)]
+    feats = np.array([S.features(d) for d in draws])
+    a = S.Surrogate(hidden=32, layers=4, max_batch=8, seed=5)
+    a.mean = rng.standard_normal(6 * S.COARSE * S.COARSE)
+    a.sd = rng.uniform(0.5, 2.0, 6 * S.COARSE * S.COARSE)
+    path = str(tmp_path / "net.npz")
+    a.save(path)
+    want = a.predict_vectors(feats)
+    a.close()
+    b = S.Surrogate(hidden=32, layers=4, max_batch=8, seed=6)
+    b.load(path)
+    got = b.predict_vectors(feats)
+    b.close()
+    assert np.array_equal(got, want)
+    c = S.Surrogate(hidden=16, layers=4, max_batch=8, seed=6)
+    with pytest.raises(ValueError):
+        c.load(path)
+    c.close()
