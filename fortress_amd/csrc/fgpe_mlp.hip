@@ -339,8 +339,7 @@ int gemm(fgpe_mlp* m, int ta, int tb, const bf16* A, int lda, const bf16* B, int
   const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   int splits = 1;
   while (tiles * splits < 256 && K / (splits * 2) >= 4 * BK) splits *= 2;
-  const size_t need = (size_t)splits * M * N;
-  if (splits > 1 && need > m->part_elems) splits = 1;
+  while (splits > 1 && (size_t)splits * M * N > m->part_elems) splits /= 2;  // partials must fit
   int kper = (K + splits - 1) / splits;
   kper = (kper + BK - 1) / BK * BK;
   splits = (K + kper - 1) / kper;
@@ -644,7 +643,9 @@ int fgpe_mlp_gemm_test(int32_t ta, int32_t tb, const uint16_t* A, const uint16_t
   };
   bf16 *dA, *dB;
   float* dC;
-  tmp.part_elems = (size_t)M * N * 16;
+  // split-K partials for up to 16 splits, at most 2^26 floats (gemm() falls
+  // back to fewer splits, or none, when they do not fit)
+  tmp.part_elems = std::min<size_t>((size_t)M * N * 16, (size_t)1 << 26);
   int rc;
   if ((rc = malloc_dev(&tmp, (void**)&dA, (size_t)M * K * sizeof(bf16))) ||
       (rc = malloc_dev(&tmp, (void**)&dB, (size_t)K * N * sizeof(bf16))) ||
@@ -680,7 +681,9 @@ int fgpe_mlp_gemm_time(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, 
   };
   bf16 *dA, *dB;
   float* dC;
-  tmp.part_elems = (size_t)M * N * 16;
+  // split-K partials for up to 16 splits, at most 2^26 floats (gemm() falls
+  // back to fewer splits, or none, when they do not fit)
+  tmp.part_elems = std::min<size_t>((size_t)M * N * 16, (size_t)1 << 26);
   int rc;
   if ((rc = malloc_dev(&tmp, (void**)&dA, (size_t)M * K * sizeof(bf16))) ||
       (rc = malloc_dev(&tmp, (void**)&dB, (size_t)K * N * sizeof(bf16))) ||
