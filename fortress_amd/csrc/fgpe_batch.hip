@@ -1,8 +1,10 @@
-// fgpe_batch.hip -- many independent q2D spin-1 instances advanced together.
+// fgpe_batch.hip -- many independent q2D instances advanced together (spin 1
+// or spin 2, one spin per batch; grids of 64, 128, 192 or 256 points per axis,
+// square or not).
 //
-// Instances are grouped by line length (64, 128, 192, 256).  Every pass of the
+// Instances are grouped by grid shape (x length, y length).  Every pass of the
 // q2D step (run_steps in fgpe_ops.hpp, SRC/cgpe2d.f90:440-485) is ONE launch
-// per length group over all of the group's instances: the grid is instances x
+// per group over all of the group's instances: the grid is instances x
 // blocks-per-instance, a block reads its instance's descriptor (state, tables,
 // parameters) and runs the same x-line or k-line block code as the
 // single-instance kernels (xpass_block / kpass_block), so a batched instance
@@ -10,12 +12,14 @@
 // renormalisation scale come from one finalize launch per group (a block per
 // instance); observables and the convergence maximum from one launch per
 // group (several blocks per instance, partial sums reduced on the host in a
-// fixed order).  A step of B instances in G length groups costs 2G launches
+// fixed order).  A step of B instances in G shape groups costs 2G launches
 // (3G in imaginary time), independent of B.
 //
 // Schedules: each line length has two (x-pass points per thread, k-pass
-// points per thread) choices; fgpe_batch_tune times both on the batch's own
-// instances and keeps the faster one per length.
+// points per thread) choices; the x pass of a group follows the choice of its
+// x-line length, the k pass that of its y-line length.  fgpe_batch_tune times
+// both choices of every length on the batch's own instances and keeps the
+// faster.
 #include <numeric>
 
 #include "fgpe_impl.hpp"
@@ -23,15 +27,15 @@
 namespace fgpe {
 
 template <typename T> struct GInst {
-  C_t<T>* phi;       // [3][n][n]
+  C_t<T>* phi;       // [ms][ny][nx]
   C_t<T>* old;       // PHI_OLD for the convergence test
   Tables<T> tb;      // twx / twy: the twiddles of the instance's schedule
   Phys ph;
   FinArgs f;
   ObsArgs oa;
   double* partials;  // NORMC partials of the x pass
-  double* norm;      // NORMC of the last step [3]
-  double* cscale64;  // renormalisation scale [3]
+  double* norm;      // NORMC of the last step [ms]
+  double* cscale64;  // renormalisation scale [ms]
   C_t<T>* cscale;    // the same in the propagation precision (= tb.cscale)
   double* sig;       // NEWTON state [2] (spin 2 only; finalize takes it)
   int* err;
@@ -39,47 +43,49 @@ template <typename T> struct GInst {
 
 constexpr int GB_OBS = 16;  // blocks per instance of the norm / observables / max-diff launches
 
-template <typename T, int N, int Q>
+// x lines of NX points (NY lines per instance)
+template <typename T, int SPIN, int NX, int Q>
 __global__ __launch_bounds__(256) void kg_xpass(const GInst<T>* __restrict__ gi, const int* __restrict__ list, int nb,
-                                                int flags) {
+                                                int ny, int flags) {
   const GInst<T>& I = gi[list[blockIdx.x / nb]];
-  const Geom g{N, N, 1, (long)N * N};
-  xpass_block<T, 1, N, Q>((int)(blockIdx.x % nb), I.phi, g, I.tb, I.ph, flags, I.partials);
+  const Geom g{NX, ny, 1, (long)NX * ny};
+  xpass_block<T, SPIN, NX, Q>((int)(blockIdx.x % nb), I.phi, g, I.tb, I.ph, flags, I.partials);
 }
 
-template <typename T, int N, int Q, int TX>
-__global__ __launch_bounds__(TX *(N / Q)) void kg_kpass(const GInst<T>* __restrict__ gi, const int* __restrict__ list,
-                                                        int nb, int scale) {
+// y lines of NY points (NX columns per instance)
+template <typename T, int SPIN, int NY, int Q, int TX>
+__global__ __launch_bounds__(TX *(NY / Q)) void kg_kpass(const GInst<T>* __restrict__ gi,
+                                                         const int* __restrict__ list, int nb, int nx, int scale) {
   // blocks (instance, x block) laid out as pass C's grid: the two 64 B halves
   // of a 128 B row go to workgroups 8 apart, on one XCD (kpass_coords)
   int bx, by;
   kpass_coords<TX * (int)sizeof(C_t<T>)>(bx, by);
   const GInst<T>& I = gi[list[bx / nb]];
-  const Geom g{N, N, 1, (long)N * N};
-  kpass_block<T, 1, N, Q, TX>(bx % nb, I.phi, g, I.tb, I.ph, scale);
+  const Geom g{nx, NY, 1, (long)nx * NY};
+  kpass_block<T, SPIN, NY, Q, TX>(bx % nb, I.phi, g, I.tb, I.ph, scale);
 }
 
-template <typename T>
+template <typename T, int MS>
 __global__ __launch_bounds__(1024) void kg_finalize(const GInst<T>* __restrict__ gi, const int* __restrict__ list,
                                                     int nparts) {
   const GInst<T>& I = gi[list[blockIdx.x]];
-  finalize_block<T, 3>(I.partials, nparts, I.f, I.norm, I.cscale, I.cscale64, I.sig, I.err);
+  finalize_block<T, MS>(I.partials, nparts, I.f, I.norm, I.cscale, I.cscale64, I.sig, I.err);
 }
 
-template <typename T> __global__ void kg_set_scale(const GInst<T>* __restrict__ gi, const int* __restrict__ list) {
+template <typename T>
+__global__ void kg_set_scale(const GInst<T>* __restrict__ gi, const int* __restrict__ list, int ms) {
   const GInst<T>& I = gi[list[blockIdx.x]];
-  if (threadIdx.x < 3) I.cscale[threadIdx.x] = mk<C_t<T>>((T)1, (T)0);
+  if ((int)threadIdx.x < ms) I.cscale[threadIdx.x] = mk<C_t<T>>((T)1, (T)0);
 }
 
 // phi *= cscale per component; blocks: instances x nb over np points each
 template <typename T>
 __global__ __launch_bounds__(256) void kg_apply_scale(const GInst<T>* __restrict__ gi, const int* __restrict__ list,
-                                                      int nb, long np) {
+                                                      int nb, long np, int ms) {
   const GInst<T>& I = gi[list[blockIdx.x / nb]];
   const long b = blockIdx.x % nb;
   for (long q = b * 256 + threadIdx.x; q < np; q += (long)nb * 256)
-#pragma unroll
-    for (int m = 0; m < 3; ++m) I.phi[m * np + q] = cmul(I.phi[m * np + q], I.cscale[m]);
+    for (int m = 0; m < ms; ++m) I.phi[m * np + q] = cmul(I.phi[m * np + q], I.cscale[m]);
 }
 
 // old = phi over n3 values
@@ -91,40 +97,42 @@ __global__ __launch_bounds__(256) void kg_keep_old(const GInst<T>* __restrict__ 
   for (long q = b * 256 + threadIdx.x; q < n3; q += (long)nb * 256) I.old[q] = I.phi[q];
 }
 
-// NORMC partials (Simpson-weighted |phi_m|^2): out[(li*GB_OBS + b)*3 + m]
-template <typename T>
-__global__ __launch_bounds__(256) void kg_norm(const GInst<T>* __restrict__ gi, const int* __restrict__ list, int n,
-                                               double* __restrict__ out) {
-  __shared__ double red[16 * 3];
+// NORMC partials (Simpson-weighted |phi_m|^2): out[(li*GB_OBS + b)*MS + m]
+template <typename T, int MS>
+__global__ __launch_bounds__(256) void kg_norm(const GInst<T>* __restrict__ gi, const int* __restrict__ list, int nx,
+                                               int ny, double* __restrict__ out) {
+  __shared__ double red[16 * MS];
   const int li = blockIdx.x / GB_OBS, b = blockIdx.x % GB_OBS;
   const GInst<T>& I = gi[list[li]];
-  const long np = (long)n * n;
-  double acc[3] = {0, 0, 0};
-  for (long q = (long)b * 256 + threadIdx.x; q < np; q += (long)GB_OBS * 256) {
-    const double w = I.oa.wx[q % n] * I.oa.wy[q / n];
+  const long np = (long)nx * ny;
+  double acc[MS];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
+  for (int m = 0; m < MS; ++m) acc[m] = 0.0;
+  for (long q = (long)b * 256 + threadIdx.x; q < np; q += (long)GB_OBS * 256) {
+    const double w = I.oa.wx[q % nx] * I.oa.wy[q / nx];
+#pragma unroll
+    for (int m = 0; m < MS; ++m) {
       const C_t<T> a = I.phi[m * np + q];
       acc[m] += w * ((double)a.x * (double)a.x + (double)a.y * (double)a.y);
     }
   }
-  block_sum_store<3>(acc, red, out, (long)blockIdx.x);
+  block_sum_store<MS>(acc, red, out, (long)blockIdx.x);
 }
 
-// ENERGY/RAD/NORMC columns: out[(li*GB_OBS + b)*11 + c]
-template <typename T>
-__global__ __launch_bounds__(256) void kg_obs(const GInst<T>* __restrict__ gi, const int* __restrict__ list, int n,
-                                              double* __restrict__ out) {
-  constexpr int NC = 11;
+// ENERGY/RAD/NORMC columns (3 MS + 2): out[(li*GB_OBS + b)*NC + c]
+template <typename T, int SPIN>
+__global__ __launch_bounds__(256) void kg_obs(const GInst<T>* __restrict__ gi, const int* __restrict__ list, int nx,
+                                              int ny, double* __restrict__ out) {
+  constexpr int NC = 3 * (2 * SPIN + 1) + 2;
   __shared__ double red[16 * NC];
   const int li = blockIdx.x / GB_OBS, b = blockIdx.x % GB_OBS;
   const GInst<T>& I = gi[list[li]];
-  const Geom g{n, n, 1, (long)n * n};
+  const Geom g{nx, ny, 1, (long)nx * ny};
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   for (long q = (long)b * 256 + threadIdx.x; q < g.npts; q += (long)GB_OBS * 256)
-    obs_point<T, 1, 2>(I.phi, g, I.oa, q, acc);
+    obs_point<T, SPIN, 2>(I.phi, g, I.oa, q, acc);
   block_sum_store<NC>(acc, red, out, (long)blockIdx.x);
 }
 
@@ -156,6 +164,7 @@ using namespace fgpe_impl;
 namespace {
 
 constexpr int NCLS = 4;
+constexpr int NGRP = NCLS * NCLS;  // shape groups: (x-length class, y-length class)
 constexpr int CLS_N[NCLS] = {64, 128, 192, 256};
 // schedule s = 2*length class + variant: points per thread of the x pass and
 // of the k pass (FP64 holds twice the registers per point: smaller Q)
@@ -167,25 +176,27 @@ constexpr int BTX = 8;  // x columns per k-pass block
 int size_class(int n) { return n == 64 ? 0 : n == 128 ? 1 : n == 192 ? 2 : n == 256 ? 3 : -1; }
 template <typename T> int sqx(int s) { return sizeof(T) == 4 ? SQX32[s] : SQX64[s]; }
 template <typename T> int sqk(int s) { return sizeof(T) == 4 ? SQK32[s] : SQK64[s]; }
-// x-pass lines per block and blocks per instance (as launch_x)
-inline int x_lpw(int n, int q) { return std::max(256 / (n / q), 1); }
-inline int x_blocks(int n, int q) { return (n + x_lpw(n, q) - 1) / x_lpw(n, q); }
+// x-pass lines per block (as launch_x) and blocks per instance of NY lines
+inline int x_lpw(int nx, int q) { return std::max(256 / (nx / q), 1); }
+inline int x_blocks(int nx, int ny, int q) { return (ny + x_lpw(nx, q) - 1) / x_lpw(nx, q); }
 
 }  // namespace
 
 struct fgpe_batch {
   int count = 0;
+  int spin = 1, ms = 3;
   bool fp32 = true;
   int dev = 0;
   size_t lds_max = 65536;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // the length groups' independent step chains run on gstream[c] (group 0 on
-  // the batch's stream), forked from and joined back to it by gev
+  // the shape groups' independent step chains run on gstream[group % 4]
+  // (gstream[0] is the batch's stream), forked from and joined back to it by gev
   hipStream_t gstream[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t gev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<fgpe_params> p;
-  std::vector<int> n, cls, sched;
+  std::vector<int> nx, ny, cx, cy;  // grid and length classes per instance
+  int csched[NCLS] = {0, 2, 4, 6};  // schedule per line length (the first variant until tuned)
   std::vector<long> off, poff;
   void* phi = nullptr;
   void* old = nullptr;
@@ -194,18 +205,20 @@ struct fgpe_batch {
   void* ginst = nullptr;  // device GInst<T>[count]
   std::vector<unsigned char> ginst_host;
   size_t ginst_size = 0;
-  double* norms = nullptr;    // [count][3]
-  double* scale64 = nullptr;  // [count][3]
-  void* cscale = nullptr;     // [count][3] complex
+  double* norms = nullptr;    // [count][ms]
+  double* scale64 = nullptr;  // [count][ms]
+  void* cscale = nullptr;     // [count][ms] complex
   int* errs = nullptr;        // [count]
-  double* obs = nullptr;      // [count][GB_OBS][12]: observables columns, then max-diff
-  int* dlist = nullptr;       // [count]: the listed instances, grouped by length
+  double* obs = nullptr;      // [count][GB_OBS][3 ms + 3]: observables columns, then max-diff
+  int* dlist = nullptr;       // [count]: the listed instances, grouped by shape
   std::vector<fgpe_batch_result> res;
   std::vector<double> en_old;
   double last_ms = 0;
   const void* twx[FGPE_BATCH_NSCHED] = {};
   const void* twk[FGPE_BATCH_NSCHED] = {};
   std::vector<void*> allocs;
+  long npts(int i) const { return (long)nx[i] * ny[i]; }
+  int ncol() const { return 3 * ms + 2; }
 };
 
 namespace {
@@ -228,12 +241,13 @@ template <typename V> int bupload(fgpe_batch* b, const V** dst, const std::vecto
 
 template <typename T> GInst<T>* host_inst(fgpe_batch* b) { return reinterpret_cast<GInst<T>*>(b->ginst_host.data()); }
 
-// point every descriptor at its schedule's twiddles and upload the array
+// point every descriptor at the twiddles of its lengths' schedules and upload
+// the array
 template <typename T> int upload_inst(fgpe_batch* b) {
   GInst<T>* in = host_inst<T>(b);
   for (int i = 0; i < b->count; ++i) {
-    in[i].tb.twx = (const R_t<T>*)b->twx[b->sched[i]];
-    in[i].tb.twy = (const R_t<T>*)b->twk[b->sched[i]];
+    in[i].tb.twx = (const R_t<T>*)b->twx[b->csched[b->cx[i]]];
+    in[i].tb.twy = (const R_t<T>*)b->twk[b->csched[b->cy[i]]];
   }
   HIPCHK(hipMemcpy(b->ginst, b->ginst_host.data(), b->ginst_size * b->count, hipMemcpyHostToDevice));
   return 0;
@@ -243,6 +257,7 @@ template <typename T> int build_batch(fgpe_batch* b) {
   b->ginst_size = sizeof(GInst<T>);
   b->ginst_host.assign(b->ginst_size * b->count, 0);
   GInst<T>* in = host_inst<T>(b);
+  const int ms = b->ms;
   int rc;
   for (int s = 0; s < FGPE_BATCH_NSCHED; ++s) {
     const R_t<T>* t;
@@ -254,19 +269,23 @@ template <typename T> int build_batch(fgpe_batch* b) {
   long ptotal = 0;
   b->poff.resize(b->count);
   for (int i = 0; i < b->count; ++i) {
-    const int c = b->cls[i];
-    const int nb = std::max(x_blocks(b->n[i], sqx<T>(2 * c)), x_blocks(b->n[i], sqx<T>(2 * c + 1)));
+    const int c = b->cx[i];
+    const int nb = std::max(x_blocks(b->nx[i], b->ny[i], sqx<T>(2 * c)), x_blocks(b->nx[i], b->ny[i], sqx<T>(2 * c + 1)));
     b->poff[i] = ptotal;
-    ptotal += (long)nb * 3;
+    ptotal += (long)nb * ms;
   }
   double *partials, *sig;
   if ((rc = balloc(b, (void**)&partials, ptotal * sizeof(double))) ||
       (rc = balloc(b, (void**)&sig, 2 * sizeof(double) * b->count)))
     return rc;
-  HIPCHK(hipMemset(sig, 0, 2 * sizeof(double) * b->count));
+  {
+    // the spin-2 Newton start (20, 20) of every instance (SRC/cgpe2d.f90:432-433)
+    std::vector<double> sig0((size_t)2 * b->count, 20.0);
+    HIPCHK(hipMemcpy(sig, sig0.data(), sig0.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   for (int i = 0; i < b->count; ++i) {
     const fgpe_params& p = b->p[i];
-    const int n = b->n[i];
+    const int nx = b->nx[i], ny = b->ny[i];
     GInst<T>& I = in[i];
     I.phi = (C<T>*)b->phi + b->off[i];
     I.old = (C<T>*)b->old + b->off[i];
@@ -278,38 +297,40 @@ template <typename T> int build_batch(fgpe_batch* b) {
     I.ph.gx = p.gamma[0];
     I.ph.gy = p.gamma[1];
     I.ph.gz = 0.0;
-    I.ph.inv_npts = 1.0 / ((double)n * n);
+    I.ph.inv_npts = 1.0 / ((double)nx * ny);
     I.ph.imag = p.imag;
     I.ph.dim = 2;
     I.f.imag = p.imag;
     I.f.soc = p.soc;
     I.f.dim = 2;
-    I.f.ms = 3;
+    I.f.ms = ms;
     I.f.mag = p.mag;
     I.f.dt = p.dt;
     I.partials = partials + b->poff[i];
-    I.norm = b->norms + 3 * i;
-    I.cscale64 = b->scale64 + 3 * i;
-    I.cscale = (C<T>*)b->cscale + 3 * i;
+    I.norm = b->norms + (size_t)ms * i;
+    I.cscale64 = b->scale64 + (size_t)ms * i;
+    I.cscale = (C<T>*)b->cscale + (size_t)ms * i;
     I.sig = sig + 2 * i;
     I.err = b->errs + i;
     // tables as build_tables (fgpe.hip) builds them for a q2D handle
-    auto kx = kgrid(n, p.d[0]), ky = kgrid(n, p.d[1]);
-    auto xx = xgrid(n, p.d[0]), yy = xgrid(n, p.d[1]);
-    std::vector<R_t<T>> kex(n), kel((size_t)3 * n);
-    for (int k = 0; k < n; ++k) kex[k] = ke_factor<T>(kx[k] * kx[k], p.dt, p.imag);
-    for (int m = 0; m < 3; ++m)
-      for (int k = 0; k < n; ++k) kel[(size_t)m * n + k] = ke_factor<T>(ky[k] * ky[k], p.dt, p.imag);
+    auto kx = kgrid(nx, p.d[0]), ky = kgrid(ny, p.d[1]);
+    auto xx = xgrid(nx, p.d[0]), yy = xgrid(ny, p.d[1]);
+    std::vector<R_t<T>> kex(nx), kel((size_t)ms * ny);
+    for (int k = 0; k < nx; ++k) kex[k] = ke_factor<T>(kx[k] * kx[k], p.dt, p.imag);
+    for (int m = 0; m < ms; ++m)
+      for (int k = 0; k < ny; ++k) kel[(size_t)m * ny + k] = ke_factor<T>(ky[k] * ky[k], p.dt, p.imag);
     std::vector<T> kxt(kx.begin(), kx.end()), kyt(ky.begin(), ky.end()), vz(1, (T)0);
-    std::vector<double> x2(n), y2(n), vxd(n), vyd(n), zero(1, 0.0), one(1, 1.0);
-    for (int k = 0; k < n; ++k) {
+    std::vector<double> x2(nx), y2(ny), vxd(nx), vyd(ny), zero(1, 0.0), one(1, 1.0);
+    for (int k = 0; k < nx; ++k) {
       x2[k] = xx[k] * xx[k];
-      y2[k] = yy[k] * yy[k];
       vxd[k] = p.alpha[0] * p.alpha[0] * x2[k];
+    }
+    for (int k = 0; k < ny; ++k) {
+      y2[k] = yy[k] * yy[k];
       vyd[k] = p.alpha[1] * p.alpha[1] * y2[k];
     }
     std::vector<T> vx(vxd.begin(), vxd.end()), vy(vyd.begin(), vyd.end());
-    auto wx = simpson_weights(n, p.d[0]), wy = simpson_weights(n, p.d[1]);
+    auto wx = simpson_weights(nx, p.d[0]), wy = simpson_weights(ny, p.d[1]);
     Tables<T>& t = I.tb;
     if ((rc = bupload(b, &t.kex, kex)) || (rc = bupload(b, &t.kel, kel)) || (rc = bupload(b, &t.kx, kxt)) ||
         (rc = bupload(b, &t.ky, kyt)) || (rc = bupload(b, &t.vx, vx)) || (rc = bupload(b, &t.vy, vy)) ||
@@ -318,7 +339,7 @@ template <typename T> int build_batch(fgpe_batch* b) {
       return rc;
     {
       const fgpe::C_t<T>* st;
-      if ((rc = bupload(b, &st, soc2_table<T>(kx, ky, p.gamma[0], p.gamma[1], p.dt, p.imag, 1)))) return rc;
+      if ((rc = bupload(b, &st, soc2_table<T>(kx, ky, p.gamma[0], p.gamma[1], p.dt, p.imag, b->spin)))) return rc;
       t.soc2 = st;
     }
     t.key = t.kex;  // not read by the 2D passes
@@ -346,20 +367,22 @@ template <typename T> int build_batch(fgpe_batch* b) {
   return upload_inst<T>(b);
 }
 
-// the listed instances grouped by length, uploaded to dlist
+// the listed instances grouped by shape (group 4 cy + cx), uploaded to dlist
 struct Groups {
-  int start[NCLS] = {}, cnt[NCLS] = {};
+  int start[NGRP] = {}, cnt[NGRP] = {};
   int total = 0;
   std::vector<int> ids;  // dlist contents
 };
+inline int gx_of(int gr) { return gr % NCLS; }
+inline int gy_of(int gr) { return gr / NCLS; }
 
 int upload_list(fgpe_batch* b, const std::vector<int>& list, Groups& g) {
   g = Groups();
-  for (int c = 0; c < NCLS; ++c) {
-    g.start[c] = (int)g.ids.size();
+  for (int gr = 0; gr < NGRP; ++gr) {
+    g.start[gr] = (int)g.ids.size();
     for (int id : list)
-      if (b->cls[id] == c) g.ids.push_back(id);
-    g.cnt[c] = (int)g.ids.size() - g.start[c];
+      if (b->cx[id] == gx_of(gr) && b->cy[id] == gy_of(gr)) g.ids.push_back(id);
+    g.cnt[gr] = (int)g.ids.size() - g.start[gr];
   }
   g.total = (int)g.ids.size();
   HIPCHK(hipStreamSynchronize(b->stream));  // the previous list is no longer read
@@ -371,103 +394,106 @@ template <typename K> void allow(K kernel, size_t bytes) {
   if (bytes > 65536) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// one x-pass launch over the cnt instances of length N at dlist + start
-template <typename T, int N, int Q> int xlaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int flags) {
-  constexpr int P = N / Q;
-  constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  const int threads = P >= 256 ? P : 256, lpw = threads / P, nb = (N + lpw - 1) / lpw;
-  const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * 3 * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
+// one x-pass launch over the cnt instances of x length NX (ny lines each) at
+// dlist + start
+template <typename T, int SPIN, int NX, int Q>
+int xlaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int ny, int flags) {
+  constexpr int P = NX / Q, MS = 2 * SPIN + 1;
+  constexpr int TWS = twiddle_size(NX, Q) > 0 ? twiddle_size(NX, Q) : 1;
+  const int threads = P >= 256 ? P : 256, lpw = threads / P, nb = (ny + lpw - 1) / lpw;
+  const size_t lds = (TWS + NX) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(NX) * sizeof(C<T>);
   if (lds > b->lds_max) return fail(FGPE_EINVAL, "batch x pass needs %zu B of LDS", lds);
-  auto kern = kg_xpass<T, N, Q>;
+  auto kern = kg_xpass<T, SPIN, NX, Q>;
   allow(kern, lds);
   hipLaunchKernelGGL(kern, dim3(cnt * nb), dim3(threads), lds, st, (const GInst<T>*)b->ginst, b->dlist + start,
-                     nb, flags);
+                     nb, ny, flags);
   HIPCHK(hipGetLastError());
   if ((flags & XF_PHYS) && (flags & XF_NORM)) {
-    hipLaunchKernelGGL(kg_finalize<T>, dim3(cnt), dim3(1024), 0, st, (const GInst<T>*)b->ginst,
+    hipLaunchKernelGGL((kg_finalize<T, MS>), dim3(cnt), dim3(1024), 0, st, (const GInst<T>*)b->ginst,
                        b->dlist + start, nb);
     HIPCHK(hipGetLastError());
   }
   return 0;
 }
-// one k-pass launch (2D: y lines, KE, SOC) over the cnt instances
-template <typename T, int N, int Q> int klaunch(fgpe_batch* b, hipStream_t st, int start, int cnt) {
-  constexpr int P = N / Q;
-  const int nb = N / BTX;
-  const size_t lds = kpass_lds_bytes<T, 1, N, Q, BTX>();
+// one k-pass launch (2D: y lines of NY points, KE, SOC) over the cnt instances
+// (nx columns each)
+template <typename T, int SPIN, int NY, int Q> int klaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int nx) {
+  constexpr int P = NY / Q;
+  static_assert(BTX * P <= 1024, "batch k pass block shape");
+  const int nb = nx / BTX;
+  const size_t lds = kpass_lds_bytes<T, SPIN, NY, Q, BTX>();
   if (lds > b->lds_max) return fail(FGPE_EINVAL, "batch k pass needs %zu B of LDS", lds);
-  auto kern = kg_kpass<T, N, Q, BTX>;
+  auto kern = kg_kpass<T, SPIN, NY, Q, BTX>;
   allow(kern, lds);
   hipLaunchKernelGGL(kern, dim3(cnt * nb), dim3(BTX * P), lds, st, (const GInst<T>*)b->ginst, b->dlist + start,
-                     nb, 1);
+                     nb, nx, 1);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 // schedule dispatch (compile-time Q per schedule)
-template <typename T> int xpass_s(fgpe_batch* b, hipStream_t st, int s, int start, int cnt, int flags) {
+template <typename T, int SPIN> int xpass_s(fgpe_batch* b, hipStream_t st, int s, int start, int cnt, int ny, int flags) {
   constexpr bool F = sizeof(T) == 4;
   switch (s) {
-    case 0: return xlaunch<T, 64, F ? SQX32[0] : SQX64[0]>(b, st, start, cnt, flags);
-    case 1: return xlaunch<T, 64, F ? SQX32[1] : SQX64[1]>(b, st, start, cnt, flags);
-    case 2: return xlaunch<T, 128, F ? SQX32[2] : SQX64[2]>(b, st, start, cnt, flags);
-    case 3: return xlaunch<T, 128, F ? SQX32[3] : SQX64[3]>(b, st, start, cnt, flags);
-    case 4: return xlaunch<T, 192, F ? SQX32[4] : SQX64[4]>(b, st, start, cnt, flags);
-    case 5: return xlaunch<T, 192, F ? SQX32[5] : SQX64[5]>(b, st, start, cnt, flags);
-    case 6: return xlaunch<T, 256, F ? SQX32[6] : SQX64[6]>(b, st, start, cnt, flags);
-    case 7: return xlaunch<T, 256, F ? SQX32[7] : SQX64[7]>(b, st, start, cnt, flags);
+    case 0: return xlaunch<T, SPIN, 64, F ? SQX32[0] : SQX64[0]>(b, st, start, cnt, ny, flags);
+    case 1: return xlaunch<T, SPIN, 64, F ? SQX32[1] : SQX64[1]>(b, st, start, cnt, ny, flags);
+    case 2: return xlaunch<T, SPIN, 128, F ? SQX32[2] : SQX64[2]>(b, st, start, cnt, ny, flags);
+    case 3: return xlaunch<T, SPIN, 128, F ? SQX32[3] : SQX64[3]>(b, st, start, cnt, ny, flags);
+    case 4: return xlaunch<T, SPIN, 192, F ? SQX32[4] : SQX64[4]>(b, st, start, cnt, ny, flags);
+    case 5: return xlaunch<T, SPIN, 192, F ? SQX32[5] : SQX64[5]>(b, st, start, cnt, ny, flags);
+    case 6: return xlaunch<T, SPIN, 256, F ? SQX32[6] : SQX64[6]>(b, st, start, cnt, ny, flags);
+    case 7: return xlaunch<T, SPIN, 256, F ? SQX32[7] : SQX64[7]>(b, st, start, cnt, ny, flags);
   }
   return fail(FGPE_EINVAL, "bad schedule %d", s);
 }
-template <typename T> int kpass_s(fgpe_batch* b, hipStream_t st, int s, int start, int cnt) {
+template <typename T, int SPIN> int kpass_s(fgpe_batch* b, hipStream_t st, int s, int start, int cnt, int nx) {
   constexpr bool F = sizeof(T) == 4;
   switch (s) {
-    case 0: return klaunch<T, 64, F ? SQK32[0] : SQK64[0]>(b, st, start, cnt);
-    case 1: return klaunch<T, 64, F ? SQK32[1] : SQK64[1]>(b, st, start, cnt);
-    case 2: return klaunch<T, 128, F ? SQK32[2] : SQK64[2]>(b, st, start, cnt);
-    case 3: return klaunch<T, 128, F ? SQK32[3] : SQK64[3]>(b, st, start, cnt);
-    case 4: return klaunch<T, 192, F ? SQK32[4] : SQK64[4]>(b, st, start, cnt);
-    case 5: return klaunch<T, 192, F ? SQK32[5] : SQK64[5]>(b, st, start, cnt);
-    case 6: return klaunch<T, 256, F ? SQK32[6] : SQK64[6]>(b, st, start, cnt);
-    case 7: return klaunch<T, 256, F ? SQK32[7] : SQK64[7]>(b, st, start, cnt);
+    case 0: return klaunch<T, SPIN, 64, F ? SQK32[0] : SQK64[0]>(b, st, start, cnt, nx);
+    case 1: return klaunch<T, SPIN, 64, F ? SQK32[1] : SQK64[1]>(b, st, start, cnt, nx);
+    case 2: return klaunch<T, SPIN, 128, F ? SQK32[2] : SQK64[2]>(b, st, start, cnt, nx);
+    case 3: return klaunch<T, SPIN, 128, F ? SQK32[3] : SQK64[3]>(b, st, start, cnt, nx);
+    case 4: return klaunch<T, SPIN, 192, F ? SQK32[4] : SQK64[4]>(b, st, start, cnt, nx);
+    case 5: return klaunch<T, SPIN, 192, F ? SQK32[5] : SQK64[5]>(b, st, start, cnt, nx);
+    case 6: return klaunch<T, SPIN, 256, F ? SQK32[6] : SQK64[6]>(b, st, start, cnt, nx);
+    case 7: return klaunch<T, SPIN, 256, F ? SQK32[7] : SQK64[7]>(b, st, start, cnt, nx);
   }
   return fail(FGPE_EINVAL, "bad schedule %d", s);
 }
-
-// the schedule of a length group (all of its instances share it)
-int group_sched(const fgpe_batch* b, const Groups& g, int c) { return b->sched[g.ids[g.start[c]]]; }
 
 // nsteps steps of every listed instance, from real space back to real space:
-// the pass schedule of run_steps (fgpe_ops.hpp) for dim 2.  Length groups
-// share nothing, so each group's chain of launches runs on its own stream:
-// the small groups' launches fill the GPU beside the large ones'.
-template <typename T> int steps_t(fgpe_batch* b, const Groups& g, long nsteps) {
+// the pass schedule of run_steps (fgpe_ops.hpp) for dim 2.  Shape groups
+// share nothing, so each group's chain of launches runs on one of four
+// streams: the small groups' launches fill the GPU beside the large ones'.
+template <typename T, int SPIN> int steps_t(fgpe_batch* b, const Groups& g, long nsteps) {
   if (nsteps <= 0 || g.total == 0) return 0;
   int rc;
-  hipLaunchKernelGGL(kg_set_scale<T>, dim3(g.total), dim3(64), 0, b->stream, (const GInst<T>*)b->ginst, b->dlist);
+  hipLaunchKernelGGL(kg_set_scale<T>, dim3(g.total), dim3(64), 0, b->stream, (const GInst<T>*)b->ginst, b->dlist,
+                     b->ms);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->gev[0], b->stream));
-  for (int c = 1; c < NCLS; ++c) HIPCHK(hipStreamWaitEvent(b->gstream[c], b->gev[0], 0));
+  for (int c = 1; c < 4; ++c) HIPCHK(hipStreamWaitEvent(b->gstream[c], b->gev[0], 0));
   const int imag = b->p[0].imag;
-  for (int c = 0; c < NCLS; ++c) {
-    if (!g.cnt[c]) continue;
-    hipStream_t st = b->gstream[c];
-    const int sc = group_sched(b, g, c);
-    if ((rc = xpass_s<T>(b, st, sc, g.start[c], g.cnt[c], XF_FWD))) return rc;
+  for (int gr = 0, k = 0; gr < NGRP; ++gr) {
+    if (!g.cnt[gr]) continue;
+    hipStream_t st = b->gstream[k++ % 4];
+    const int nx = CLS_N[gx_of(gr)], ny = CLS_N[gy_of(gr)];
+    const int sx = b->csched[gx_of(gr)], sk = b->csched[gy_of(gr)];
+    if ((rc = xpass_s<T, SPIN>(b, st, sx, g.start[gr], g.cnt[gr], ny, XF_FWD))) return rc;
     for (long s = 1; s <= nsteps; ++s) {
-      if ((rc = kpass_s<T>(b, st, sc, g.start[c], g.cnt[c]))) return rc;
+      if ((rc = kpass_s<T, SPIN>(b, st, sk, g.start[gr], g.cnt[gr], nx))) return rc;
       const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((imag || s == nsteps) ? XF_NORM : 0);
-      if ((rc = xpass_s<T>(b, st, sc, g.start[c], g.cnt[c], flags))) return rc;
+      if ((rc = xpass_s<T, SPIN>(b, st, sx, g.start[gr], g.cnt[gr], ny, flags))) return rc;
     }
     if (imag) {
-      const long np = (long)CLS_N[c] * CLS_N[c];
+      const long np = (long)nx * ny;
       const int nb = (int)std::min<long>((np + 255) / 256, 64);
-      hipLaunchKernelGGL(kg_apply_scale<T>, dim3(g.cnt[c] * nb), dim3(256), 0, st, (const GInst<T>*)b->ginst,
-                         b->dlist + g.start[c], nb, np);
+      hipLaunchKernelGGL(kg_apply_scale<T>, dim3(g.cnt[gr] * nb), dim3(256), 0, st, (const GInst<T>*)b->ginst,
+                         b->dlist + g.start[gr], nb, np, b->ms);
       HIPCHK(hipGetLastError());
     }
   }
-  for (int c = 1; c < NCLS; ++c) {
+  for (int c = 1; c < 4; ++c) {
     HIPCHK(hipEventRecord(b->gev[c], b->gstream[c]));
     HIPCHK(hipStreamWaitEvent(b->stream, b->gev[c], 0));
   }
@@ -475,29 +501,32 @@ template <typename T> int steps_t(fgpe_batch* b, const Groups& g, long nsteps) {
 }
 
 template <typename T> int keep_old_t(fgpe_batch* b, const Groups& g) {
-  for (int c = 0; c < NCLS; ++c)
-    if (g.cnt[c]) {
-      const long n3 = 3L * CLS_N[c] * CLS_N[c];
+  for (int gr = 0; gr < NGRP; ++gr)
+    if (g.cnt[gr]) {
+      const long n3 = (long)b->ms * CLS_N[gx_of(gr)] * CLS_N[gy_of(gr)];
       const int nb = (int)std::min<long>((n3 + 255) / 256, 64);
-      hipLaunchKernelGGL(kg_keep_old<T>, dim3(g.cnt[c] * nb), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
-                         b->dlist + g.start[c], nb, n3);
+      hipLaunchKernelGGL(kg_keep_old<T>, dim3(g.cnt[gr] * nb), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
+                         b->dlist + g.start[gr], nb, n3);
       HIPCHK(hipGetLastError());
     }
   return 0;
 }
 
 // observables (and the convergence maximum) of the listed instances into res
-template <typename T> int events_t(fgpe_batch* b, const Groups& g, bool conv) {
-  const size_t ncol = (size_t)g.total * GB_OBS * 11;
+template <typename T, int SPIN> int events_t(fgpe_batch* b, const Groups& g, bool conv) {
+  constexpr int MS = 2 * SPIN + 1, NC = 3 * MS + 2;
+  const size_t ncol = (size_t)g.total * GB_OBS * NC;
   std::vector<double> part(ncol), md((size_t)g.total * GB_OBS);
-  for (int c = 0; c < NCLS; ++c)
-    if (g.cnt[c]) {
-      hipLaunchKernelGGL(kg_obs<T>, dim3(g.cnt[c] * GB_OBS), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
-                         b->dlist + g.start[c], CLS_N[c], b->obs + (size_t)g.start[c] * GB_OBS * 11);
+  for (int gr = 0; gr < NGRP; ++gr)
+    if (g.cnt[gr]) {
+      const int nx = CLS_N[gx_of(gr)], ny = CLS_N[gy_of(gr)];
+      hipLaunchKernelGGL((kg_obs<T, SPIN>), dim3(g.cnt[gr] * GB_OBS), dim3(256), 0, b->stream,
+                         (const GInst<T>*)b->ginst, b->dlist + g.start[gr], nx, ny,
+                         b->obs + (size_t)g.start[gr] * GB_OBS * NC);
       HIPCHK(hipGetLastError());
       if (conv) {
-        hipLaunchKernelGGL(kg_maxdiff<T>, dim3(g.cnt[c] * GB_OBS), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
-                           b->dlist + g.start[c], 3L * CLS_N[c] * CLS_N[c], b->obs + ncol + (size_t)g.start[c] * GB_OBS);
+        hipLaunchKernelGGL(kg_maxdiff<T>, dim3(g.cnt[gr] * GB_OBS), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
+                           b->dlist + g.start[gr], (long)MS * nx * ny, b->obs + ncol + (size_t)g.start[gr] * GB_OBS);
         HIPCHK(hipGetLastError());
       }
     }
@@ -506,18 +535,18 @@ template <typename T> int events_t(fgpe_batch* b, const Groups& g, bool conv) {
   if (conv) HIPCHK(hipMemcpy(md.data(), b->obs + ncol, md.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int li = 0; li < g.total; ++li) {
     fgpe_batch_result& r = b->res[g.ids[li]];
-    double s[11] = {0};
+    double s[NC] = {0};
     for (int k = 0; k < GB_OBS; ++k)
-      for (int c = 0; c < 11; ++c) s[c] += part[((size_t)li * GB_OBS + k) * 11 + c];
+      for (int c = 0; c < NC; ++c) s[c] += part[((size_t)li * GB_OBS + k) * NC + c];
     double sumn = 0;
-    for (int m = 0; m < 3; ++m) sumn += s[2 * 3 + 2 + m];
+    for (int m = 0; m < MS; ++m) sumn += s[2 * MS + 2 + m];
     // EN = int col1 / sum(NORM), MU = int col / (NORM + EPS), RMS = sqrt(int
-    // r^2 |phi|^2) (SRC/cgpe2d.f90:1276-1279, 1069), MZ = int (|phi_1|^2 - |phi_3|^2)
+    // r^2 |phi|^2) (SRC/cgpe2d.f90:1276-1279, 1069), MZ = int sum m |phi_m|^2
     r.en = s[0] / sumn;
     r.mz = s[1];
-    for (int m = 0; m < 3; ++m) {
-      r.mu[m] = s[2 + m] / (s[2 * 3 + 2 + m] + EPS);
-      r.rms[m] = std::sqrt(s[3 + 2 + m]);
+    for (int m = 0; m < MS; ++m) {
+      r.mu[m] = s[2 + m] / (s[2 * MS + 2 + m] + EPS);
+      r.rms[m] = std::sqrt(s[MS + 2 + m]);
     }
     if (conv) {
       double mx = 0;
@@ -529,12 +558,13 @@ template <typename T> int events_t(fgpe_batch* b, const Groups& g, bool conv) {
 }
 
 int pull_norms(fgpe_batch* b, const Groups& g) {
-  std::vector<double> nn((size_t)b->count * 3);
+  const int ms = b->ms;
+  std::vector<double> nn((size_t)b->count * ms);
   std::vector<int> er(b->count);
   HIPCHK(hipMemcpy(nn.data(), b->norms, nn.size() * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(er.data(), b->errs, er.size() * sizeof(int), hipMemcpyDeviceToHost));
   for (int id : g.ids) {
-    for (int m = 0; m < 3; ++m) b->res[id].norm[m] = nn[(size_t)id * 3 + m];
+    for (int m = 0; m < ms; ++m) b->res[id].norm[m] = nn[(size_t)id * ms + m];
     if (er[id]) b->res[id].err = 1;
   }
   return 0;
@@ -543,18 +573,20 @@ int pull_norms(fgpe_batch* b, const Groups& g) {
 // advance the listed instances by nsteps (PHI_OLD kept one step before the
 // end when conv), then their observables; device time of the steps into
 // last_ms when timed
-template <typename T> int advance_t(fgpe_batch* b, const std::vector<int>& list, long nsteps, bool conv, bool timed) {
+template <typename T, int SPIN>
+int advance_t(fgpe_batch* b, const std::vector<int>& list, long nsteps, bool conv, bool timed) {
   Groups g;
   int rc = upload_list(b, list, g);
   if (rc || g.total == 0) return rc;
   if (timed) HIPCHK(hipEventRecord(b->ev0, b->stream));
   if (conv && nsteps >= 1) {
-    if ((rc = steps_t<T>(b, g, nsteps - 1)) || (rc = keep_old_t<T>(b, g)) || (rc = steps_t<T>(b, g, 1))) return rc;
-  } else if ((rc = steps_t<T>(b, g, nsteps))) {
+    if ((rc = steps_t<T, SPIN>(b, g, nsteps - 1)) || (rc = keep_old_t<T>(b, g)) || (rc = steps_t<T, SPIN>(b, g, 1)))
+      return rc;
+  } else if ((rc = steps_t<T, SPIN>(b, g, nsteps))) {
     return rc;
   }
   if (timed) HIPCHK(hipEventRecord(b->ev1, b->stream));
-  if ((rc = events_t<T>(b, g, conv))) return rc;
+  if ((rc = events_t<T, SPIN>(b, g, conv))) return rc;
   if (timed) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -566,7 +598,9 @@ template <typename T> int advance_t(fgpe_batch* b, const std::vector<int>& list,
 }
 
 int advance(fgpe_batch* b, const std::vector<int>& list, long nsteps, bool conv, bool timed) {
-  return b->fp32 ? advance_t<float>(b, list, nsteps, conv, timed) : advance_t<double>(b, list, nsteps, conv, timed);
+  if (b->spin == 1)
+    return b->fp32 ? advance_t<float, 1>(b, list, nsteps, conv, timed) : advance_t<double, 1>(b, list, nsteps, conv, timed);
+  return b->fp32 ? advance_t<float, 2>(b, list, nsteps, conv, timed) : advance_t<double, 2>(b, list, nsteps, conv, timed);
 }
 
 std::vector<int> all_ids(const fgpe_batch* b) {
@@ -576,44 +610,50 @@ std::vector<int> all_ids(const fgpe_batch* b) {
 }
 
 // NORMT of every instance (as fgpe_normalize): total norm 1
-template <typename T> int normt_t(fgpe_batch* b) {
+template <typename T, int MS> int normt_t(fgpe_batch* b) {
   Groups g;
   int rc = upload_list(b, all_ids(b), g);
   if (rc) return rc;
-  std::vector<double> part((size_t)g.total * GB_OBS * 3);
-  for (int c = 0; c < NCLS; ++c)
-    if (g.cnt[c]) {
-      hipLaunchKernelGGL(kg_norm<T>, dim3(g.cnt[c] * GB_OBS), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
-                         b->dlist + g.start[c], CLS_N[c], b->obs + (size_t)g.start[c] * GB_OBS * 3);
+  std::vector<double> part((size_t)g.total * GB_OBS * MS);
+  for (int gr = 0; gr < NGRP; ++gr)
+    if (g.cnt[gr]) {
+      hipLaunchKernelGGL((kg_norm<T, MS>), dim3(g.cnt[gr] * GB_OBS), dim3(256), 0, b->stream,
+                         (const GInst<T>*)b->ginst, b->dlist + g.start[gr], CLS_N[gx_of(gr)], CLS_N[gy_of(gr)],
+                         b->obs + (size_t)g.start[gr] * GB_OBS * MS);
       HIPCHK(hipGetLastError());
     }
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(part.data(), b->obs, part.size() * sizeof(double), hipMemcpyDeviceToHost));
-  std::vector<C<T>> sc((size_t)b->count * 3);
+  std::vector<C<T>> sc((size_t)b->count * MS);
   for (int li = 0; li < g.total; ++li) {
     double t = 0;
-    for (int m = 0; m < 3; ++m) {
+    for (int m = 0; m < MS; ++m) {
       double s = 0;
-      for (int k = 0; k < GB_OBS; ++k) s += part[((size_t)li * GB_OBS + k) * 3 + m];
+      for (int k = 0; k < GB_OBS; ++k) s += part[((size_t)li * GB_OBS + k) * MS + m];
       t += s;
     }
     const T c = (T)(1.0 / std::sqrt(t));
-    for (int m = 0; m < 3; ++m) {
-      sc[(size_t)g.ids[li] * 3 + m].x = c;
-      sc[(size_t)g.ids[li] * 3 + m].y = (T)0;
+    for (int m = 0; m < MS; ++m) {
+      sc[(size_t)g.ids[li] * MS + m].x = c;
+      sc[(size_t)g.ids[li] * MS + m].y = (T)0;
     }
   }
   HIPCHK(hipMemcpy(b->cscale, sc.data(), sc.size() * sizeof(C<T>), hipMemcpyHostToDevice));
-  for (int c = 0; c < NCLS; ++c)
-    if (g.cnt[c]) {
-      const long np = (long)CLS_N[c] * CLS_N[c];
+  for (int gr = 0; gr < NGRP; ++gr)
+    if (g.cnt[gr]) {
+      const long np = (long)CLS_N[gx_of(gr)] * CLS_N[gy_of(gr)];
       const int nb = (int)std::min<long>((np + 255) / 256, 64);
-      hipLaunchKernelGGL(kg_apply_scale<T>, dim3(g.cnt[c] * nb), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
-                         b->dlist + g.start[c], nb, np);
+      hipLaunchKernelGGL(kg_apply_scale<T>, dim3(g.cnt[gr] * nb), dim3(256), 0, b->stream, (const GInst<T>*)b->ginst,
+                         b->dlist + g.start[gr], nb, np, MS);
       HIPCHK(hipGetLastError());
     }
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
+}
+
+int normt(fgpe_batch* b) {
+  if (b->spin == 1) return b->fp32 ? normt_t<float, 3>(b) : normt_t<double, 3>(b);
+  return b->fp32 ? normt_t<float, 5>(b) : normt_t<double, 5>(b);
 }
 
 }  // namespace
@@ -626,9 +666,11 @@ int fgpe_batch_create(const fgpe_params* p, int32_t count, fgpe_batch** out) {
   *out = nullptr;
   for (int i = 0; i < count; ++i) {
     if (p[i].dim != 2) return fail(FGPE_EINVAL, "batch instance %d: only q2D instances are batched", i);
-    if (p[i].spin != 1) return fail(FGPE_EINVAL, "batch instance %d: only spin-1 instances are batched", i);
-    if (p[i].n[0] != p[i].n[1] || size_class(p[i].n[0]) < 0)
-      return fail(FGPE_EINVAL, "batch instance %d: grid %dx%d (square 64, 128, 192 or 256)", i, p[i].n[0], p[i].n[1]);
+    if (p[i].spin != 1 && p[i].spin != 2) return fail(FGPE_EINVAL, "batch instance %d: spin %d", i, p[i].spin);
+    if (p[i].spin != p[0].spin) return fail(FGPE_EINVAL, "batch instance %d: mixed spin", i);
+    if (size_class(p[i].n[0]) < 0 || size_class(p[i].n[1]) < 0)
+      return fail(FGPE_EINVAL, "batch instance %d: grid %dx%d (64, 128, 192 or 256 points per axis)", i, p[i].n[0],
+                  p[i].n[1]);
     if (p[i].precision != p[0].precision) return fail(FGPE_EINVAL, "batch instance %d: mixed precision", i);
     if (p[i].imag != p[0].imag) return fail(FGPE_EINVAL, "batch instance %d: mixed real and imaginary time", i);
     if (!(p[i].dt > 0) || !(p[i].d[0] > 0) || !(p[i].d[1] > 0)) return fail(FGPE_EINVAL, "batch instance %d: bad DT/DX", i);
@@ -641,6 +683,8 @@ int fgpe_batch_create(const fgpe_params* p, int32_t count, fgpe_batch** out) {
     return rc;
   };
   b->count = count;
+  b->spin = p[0].spin;
+  b->ms = 2 * b->spin + 1;
   b->fp32 = p[0].precision == FGPE_FP32;
   b->p.assign(p, p + count);
   if (hipGetDevice(&b->dev) != hipSuccess) return bail(fail(FGPE_EDEVICE, "hipGetDevice failed"));
@@ -658,32 +702,34 @@ int fgpe_batch_create(const fgpe_params* p, int32_t count, fgpe_batch** out) {
       return bail(fail(FGPE_EDEVICE, "hipStreamCreate/hipEventCreate failed"));
   }
   const size_t es = b->fp32 ? sizeof(float2) : sizeof(double2);
+  const int ms = b->ms;
   long total = 0;
-  b->n.resize(count);
-  b->cls.resize(count);
-  b->sched.resize(count);
+  b->nx.resize(count);
+  b->ny.resize(count);
+  b->cx.resize(count);
+  b->cy.resize(count);
   b->off.resize(count);
   for (int i = 0; i < count; ++i) {
-    const int n = p[i].n[0];
-    b->n[i] = n;
-    b->cls[i] = size_class(n);
-    b->sched[i] = 2 * b->cls[i];  // the first schedule of the length until tuned
+    b->nx[i] = p[i].n[0];
+    b->ny[i] = p[i].n[1];
+    b->cx[i] = size_class(b->nx[i]);
+    b->cy[i] = size_class(b->ny[i]);
     b->off[i] = total;
-    total += (long)n * n * 3;
-    b->stage_elems = std::max(b->stage_elems, (size_t)n * n * 3);
+    total += b->npts(i) * ms;
+    b->stage_elems = std::max(b->stage_elems, (size_t)b->npts(i) * ms);
   }
   int rc;
   if ((rc = balloc(b, &b->phi, total * es)) || (rc = balloc(b, &b->old, total * es)) ||
       (rc = balloc(b, &b->stage, b->stage_elems * sizeof(double2))) ||
-      (rc = balloc(b, (void**)&b->norms, 3 * sizeof(double) * count)) ||
-      (rc = balloc(b, (void**)&b->scale64, 3 * sizeof(double) * count)) ||
-      (rc = balloc(b, &b->cscale, 3 * es * count)) || (rc = balloc(b, (void**)&b->errs, sizeof(int) * count)) ||
-      (rc = balloc(b, (void**)&b->obs, sizeof(double) * (size_t)count * GB_OBS * 12)) ||
+      (rc = balloc(b, (void**)&b->norms, ms * sizeof(double) * count)) ||
+      (rc = balloc(b, (void**)&b->scale64, ms * sizeof(double) * count)) ||
+      (rc = balloc(b, &b->cscale, ms * es * count)) || (rc = balloc(b, (void**)&b->errs, sizeof(int) * count)) ||
+      (rc = balloc(b, (void**)&b->obs, sizeof(double) * (size_t)count * GB_OBS * (b->ncol() + 1))) ||
       (rc = balloc(b, (void**)&b->dlist, sizeof(int) * count)))
     return bail(rc);
   HIPCHK(hipMemset(b->phi, 0, total * es));
   HIPCHK(hipMemset(b->old, 0, total * es));
-  HIPCHK(hipMemset(b->norms, 0, 3 * sizeof(double) * count));
+  HIPCHK(hipMemset(b->norms, 0, ms * sizeof(double) * count));
   HIPCHK(hipMemset(b->errs, 0, sizeof(int) * count));
   b->res.assign(count, fgpe_batch_result{});
   b->en_old.assign(count, 0.0);
@@ -714,10 +760,9 @@ int fgpe_batch_initial_guess(fgpe_batch* b) {
   if (!b) return fail(FGPE_EINVAL, "null batch");
   for (int i = 0; i < b->count; ++i) {
     const fgpe_params& p = b->p[i];
-    const int n = b->n[i];
     GuessArgs a;
     a.dim = 2;
-    a.spin = 1;
+    a.spin = b->spin;
     a.fpc = p.option_fpc;
     a.ax = p.alpha[0];
     a.ay = p.alpha[1];
@@ -726,7 +771,7 @@ int fgpe_batch_initial_guess(fgpe_batch* b) {
     a.t1 = p.tau[1];
     a.t2 = p.tau[2];
     a.mag = p.mag;
-    const Geom g{n, n, 1, (long)n * n};
+    const Geom g{b->nx[i], b->ny[i], 1, b->npts(i)};
     const int blocks = std::min(grid_for(g.npts, 256), 1024);
     if (b->fp32) {
       const GInst<float>& I = host_inst<float>(b)[i];
@@ -744,7 +789,7 @@ int fgpe_batch_initial_guess(fgpe_batch* b) {
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(b->stream));
-  int rc = b->fp32 ? normt_t<float>(b) : normt_t<double>(b);
+  int rc = normt(b);
   if (rc) return rc;
   for (auto& r : b->res) r = fgpe_batch_result{};
   if ((rc = advance(b, all_ids(b), 0, false, false))) return rc;
@@ -754,7 +799,7 @@ int fgpe_batch_initial_guess(fgpe_batch* b) {
 
 int fgpe_batch_normalize(fgpe_batch* b) {
   if (!b) return fail(FGPE_EINVAL, "null batch");
-  int rc = b->fp32 ? normt_t<float>(b) : normt_t<double>(b);
+  int rc = normt(b);
   if (rc) return rc;
   for (auto& r : b->res) r = fgpe_batch_result{};
   if ((rc = advance(b, all_ids(b), 0, false, false))) return rc;
@@ -764,7 +809,7 @@ int fgpe_batch_normalize(fgpe_batch* b) {
 
 int fgpe_batch_upload(fgpe_batch* b, int32_t i, const double* phi) {
   if (!b || !phi || i < 0 || i >= b->count) return fail(FGPE_EINVAL, "bad batch upload argument");
-  const long n = (long)b->n[i] * b->n[i] * 3;
+  const long n = b->npts(i) * b->ms;
   HIPCHK(hipStreamSynchronize(b->stream));
   if (!b->fp32) {
     HIPCHK(hipMemcpy((double2*)b->phi + b->off[i], phi, n * sizeof(double2), hipMemcpyHostToDevice));
@@ -780,7 +825,7 @@ int fgpe_batch_upload(fgpe_batch* b, int32_t i, const double* phi) {
 
 int fgpe_batch_download(fgpe_batch* b, int32_t i, double* phi) {
   if (!b || !phi || i < 0 || i >= b->count) return fail(FGPE_EINVAL, "bad batch download argument");
-  const long n = (long)b->n[i] * b->n[i] * 3;
+  const long n = b->npts(i) * b->ms;
   HIPCHK(hipStreamSynchronize(b->stream));
   if (!b->fp32) {
     HIPCHK(hipMemcpy(phi, (double2*)b->phi + b->off[i], n * sizeof(double2), hipMemcpyDeviceToHost));
@@ -842,7 +887,7 @@ int fgpe_batch_tune(fgpe_batch* b, int32_t trial_steps, double* ms_out, int32_t*
   if (!b || trial_steps < 1) return fail(FGPE_EINVAL, "bad batch tune argument");
   const size_t es = b->fp32 ? sizeof(float2) : sizeof(double2);
   long total = 0;
-  for (int i = 0; i < b->count; ++i) total += (long)b->n[i] * b->n[i] * 3;
+  for (int i = 0; i < b->count; ++i) total += b->npts(i) * b->ms;
   // trials advance the state: keep a copy and restore it afterwards
   void* backup = nullptr;
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -857,14 +902,20 @@ int fgpe_batch_tune(fgpe_batch* b, int32_t trial_steps, double* ms_out, int32_t*
   double ms[FGPE_BATCH_NSCHED] = {0};
   int best[NCLS] = {-1, -1, -1, -1};
   int rc;
+  // one line length at a time: its two schedules timed on the instances of
+  // its square group, or, without one, on every instance with lines of that
+  // length (the other length keeping its current schedule)
   for (int c = 0; c < NCLS; ++c) {
-    std::vector<int> list;
-    for (int i = 0; i < b->count; ++i)
-      if (b->cls[i] == c) list.push_back(i);
+    std::vector<int> sq, any;
+    for (int i = 0; i < b->count; ++i) {
+      if (b->cx[i] == c && b->cy[i] == c) sq.push_back(i);
+      if (b->cx[i] == c || b->cy[i] == c) any.push_back(i);
+    }
+    const std::vector<int>& list = sq.empty() ? any : sq;
     if (list.empty()) continue;
     double best_ms = 1e300;
     for (int s = 2 * c; s < 2 * c + 2; ++s) {
-      for (int id : list) b->sched[id] = s;
+      b->csched[c] = s;
       if ((rc = b->fp32 ? upload_inst<float>(b) : upload_inst<double>(b))) return done(rc);
       // one untimed step to warm up, then the timed trial
       if ((rc = advance(b, list, 1, false, false)) || (rc = advance(b, list, trial_steps, false, true))) return done(rc);
@@ -874,7 +925,7 @@ int fgpe_batch_tune(fgpe_batch* b, int32_t trial_steps, double* ms_out, int32_t*
         best[c] = s;
       }
     }
-    for (int id : list) b->sched[id] = best[c];
+    b->csched[c] = best[c];
   }
   if ((rc = b->fp32 ? upload_inst<float>(b) : upload_inst<double>(b))) return done(rc);
   HIPCHK(hipStreamSynchronize(b->stream));

@@ -316,7 +316,8 @@ class Solver:
 
 
 class Batch:
-    """Independent q2D spin-1 instances advanced together (fgpe_batch_*)."""
+    """Independent q2D instances advanced together (fgpe_batch_*): spin 1 or
+    2 (one spin per batch), 64-256 points per axis."""
 
     def __init__(self, params_list):
         arr = (FgpeParams * len(params_list))(*params_list)
@@ -345,7 +346,7 @@ class Batch:
 
     def shape(self, i):
         p = self.params[i]
-        return (3, 1, p.n[1], p.n[0])
+        return (2 * p.spin + 1, 1, p.n[1], p.n[0])
 
     def initial_guess(self):
         _check(lib().fgpe_batch_initial_guess(self.h), "fgpe_batch_initial_guess")

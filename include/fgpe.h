@@ -287,8 +287,9 @@ int fgpe_last_error(char* buf, size_t len);
  * grouped by grid size and every pass of a step is one launch per size over
  * all of that size's unconverged instances (blocks map to instance x lines),
  * so a step of any number of instances costs a fixed handful of launches.
- * Instances: dim 2, spin 1, square grids of 64, 128, 192 or 256 points per
- * axis, one precision and one time mode (real or imaginary) for the batch.
+ * Instances: dim 2, spin 1 or 2, 64, 128, 192 or 256 points along each axis
+ * (square or not); one spin, one precision and one time mode (real or
+ * imaginary) for the batch.  Instances are grouped by grid shape.
  */
 typedef struct fgpe_batch fgpe_batch;
 
@@ -306,8 +307,9 @@ typedef struct fgpe_batch_result {
   double norm[5];   /* NORMC of the last step, before renormalisation */
 } fgpe_batch_result;
 
-/* Schedules (points per thread of the x pass / the k pass), two per grid
- * size: s = 2*c + v for c = 0..3 (64, 128, 192, 256).  FP32: 64 (4/8, 8/4),
+/* Schedules (points per thread of the x pass / the k pass), two per line
+ * length: s = 2*c + v for c = 0..3 (64, 128, 192, 256).  An instance's x pass
+ * follows the schedule of its x length, its k pass that of its y length.  FP32: 64 (4/8, 8/4),
  * 128 (4/8, 8/16), 192 (12/12, 6/6), 256 (4/8, 8/16); FP64: 64 (8/4, 4/8),
  * 128 (8/4, 4/8), 192 (6/6, 12/6), 256 (8/4, 4/8). */
 #define FGPE_BATCH_NSCHED 8
@@ -344,9 +346,10 @@ int fgpe_batch_solve(fgpe_batch* b, int32_t stp, double tol, int64_t max_steps, 
 /* Result of instance i. */
 int fgpe_batch_result_get(fgpe_batch* b, int32_t i, fgpe_batch_result* r);
 
-/* Autotuner: for every line length present, time trial_steps steps of that
- * length's instances under each of its schedules (state restored
- * afterwards) and keep the fastest.  ms_out[FGPE_BATCH_NSCHED] receives the
+/* Autotuner: for every line length present, time trial_steps steps of the
+ * instances of that square shape (or, with none, of every instance with lines
+ * of that length) under each of its schedules (state restored afterwards)
+ * and keep the fastest.  ms_out[FGPE_BATCH_NSCHED] receives the
  * measured milliseconds per schedule (0 where not timed), chosen[4] the
  * schedule kept for 64, 128, 192 and 256 points (-1 where absent). */
 int fgpe_batch_tune(fgpe_batch* b, int32_t trial_steps, double* ms_out, int32_t* chosen);

@@ -6,20 +6,26 @@ from fortress_amd import fgpe
 from fortress_amd.abi import FGPE_FP32, FGPE_FP64, make_input
 
 
-def params(n=64, spin=1, precision=FGPE_FP32, imag=1):
-    inp = make_input(2, spin, (n, n), (0.1, 0.1), 1e-3, 87.0, (101.8, 100.4, 0.0), 10000, (20.0, 20.0, 400.0),
-                     (0.5, 0.5), switch_im=imag)
+def params(n=64, spin=1, precision=FGPE_FP32, imag=1, ny=None, dim=2):
+    a = (101.8, 100.4, 0.0) if spin == 1 else (87.93, 91.28, 99.18)
+    grid = (n, ny or n) if dim == 2 else (n, n, n)
+    inp = make_input(dim, spin, grid, (0.1,) * dim, 1e-3, 87.0, a, 10000, (20.0, 20.0, 400.0),
+                     (0.5,) * dim, switch_im=imag)
     return fgpe.derive(inp, precision)[0]
 
 
 def test_grid_sizes_outside_the_batched_set_are_refused():
-    with pytest.raises(fgpe.FgpeError, match="square 64, 128, 192 or 256"):
+    with pytest.raises(fgpe.FgpeError, match="64, 128, 192 or 256 points per axis"):
         fgpe.Batch([params(64), params(96)])
+    with pytest.raises(fgpe.FgpeError, match="64, 128, 192 or 256 points per axis"):
+        fgpe.Batch([params(64, ny=32)])
 
 
-def test_spin2_instances_are_refused():
-    with pytest.raises(fgpe.FgpeError, match="spin-1"):
-        fgpe.Batch([params(spin=2)])
+def test_mixed_spin_and_other_dimensions_are_refused():
+    with pytest.raises(fgpe.FgpeError, match="mixed spin"):
+        fgpe.Batch([params(spin=1), params(spin=2)])
+    with pytest.raises(fgpe.FgpeError, match="only q2D"):
+        fgpe.Batch([params(32, dim=3)])
 
 
 def test_mixed_precision_and_time_mode_are_refused():

@@ -1,12 +1,14 @@
-"""Batched q2D spin-1 instances (fgpe_batch_*) against the FP64 oracle and the
+"""Batched q2D instances (fgpe_batch_*) against the FP64 oracle and the
 single-instance path.
 
 Instances follow the parameter sweep of the batch workload (square grids of
 64, 128, 192 and 256 points, DX = 0.1, DT = 0.001, spin-1 87Rb with A2 - A0
 varied, GAMMAX/GAMMAY, NATOMS and the in-plane trap aspect varied, imaginary
-time).  Bars: FP64 1e-10 relative on phi, norms and observables; FP32
-densities 1e-5.  Each instance's result must not depend on what else is in
-the batch or on the schedule the autotuner picks (FP64: 1e-12).
+time), plus rectangular grids and spin-2 87Rb batches (with and without SOC:
+the SOC = 0 instances take the Newton projection).  Bars: FP64 1e-10
+relative on phi, norms and observables; FP32 densities 1e-5.  Each
+instance's result must not depend on what else is in the batch or on the
+schedule the autotuner picks (FP64: 1e-12).
 """
 import numpy as np
 import pytest
@@ -31,6 +33,25 @@ def inst(n, gx, gy, da, natoms, aspect):
                       (20.0, 20.0 * aspect, 400.0), (gx, gy), switch_im=1)
 
 
+def inst_shape(nx, ny, spin, gx, gy, natoms, aspect, mag=0.0):
+    a = (87.93, 91.28, 99.18) if spin == 2 else (101.8, 103.3, 0.0)
+    return make_input(2, spin, (nx, ny), (0.1, 0.1), 1e-3, 87.0, a, natoms, (20.0, 20.0 * aspect, 400.0), (gx, gy),
+                      switch_im=1, mag=mag)
+
+
+SPIN2_SHAPES = [  # nx, ny, spin, GAMMAX, GAMMAY, NATOMS, aspect, MAG
+    (64, 64, 2, 1.0, 1.0, 50000, 1.0, 0.0),
+    (128, 64, 2, 0.6, 1.3, 20000, 1.1, 0.0),
+    (64, 192, 2, 1.4, 0.3, 80000, 0.9, 0.0),
+    (128, 128, 2, 0.0, 0.0, 30000, 1.0, 0.3),
+]
+SPIN1_SHAPES = [
+    (64, 128, 1, 0.3, 1.7, 2000, 0.9),
+    (256, 64, 1, 1.2, 0.4, 30000, 1.2),
+    (192, 128, 1, 0.8, 0.8, 10000, 1.0),
+]
+
+
 def oracle_start(inp):
     p, _ = O.derive(inp)
     o = O.Oracle(p)
@@ -39,16 +60,15 @@ def oracle_start(inp):
     return o
 
 
-def test_batch_fp64_matches_oracle():
-    inps = [inst(*s) for s in SWEEP]
+def check_fp64_against_oracle(inps, steps=20):
     params = [fgpe.derive(i, FGPE_FP64)[0] for i in inps]
-    steps = 20
     with fgpe.Batch(params) as b:
         b.initial_guess()
         starts = [b.download(i) for i in range(len(inps))]
         b.run(steps)
         for i, inp in enumerate(inps):
-            print("instance", i, "n", SWEEP[i][0], flush=True)
+            ms = 2 * params[i].spin + 1
+            print("instance", i, "grid", tuple(params[i].n[:2]), "spin", params[i].spin, flush=True)
             o = oracle_start(inp)
             assert rel_l2(starts[i], o.get_phi()) < 1e-13  # guess + NORMT
             o.run(steps - 1)
@@ -60,16 +80,67 @@ def test_batch_fp64_matches_oracle():
             err = rel_l2(got, ref)
             print("  rel_l2 phi", err, flush=True)
             assert err < 1e-10
-            assert r.steps == steps
-            assert np.allclose(np.array(r.norm[:3]), o.last_norms(), rtol=1e-10, atol=1e-14)
+            assert r.steps == steps and r.err == 0
+            assert np.allclose(np.array(r.norm[:ms]), o.last_norms(), rtol=1e-10, atol=1e-14)
             en, mu, mz = o.energy()
             assert abs(r.en - en) <= 1e-10 * abs(en)
-            assert np.allclose(np.array(r.mu[:3]), mu, rtol=1e-10, atol=1e-12)
+            assert np.allclose(np.array(r.mu[:ms]), mu, rtol=1e-10, atol=1e-12)
             assert abs(r.mz - mz) <= 1e-10 * max(1.0, abs(mz))
-            assert np.allclose(np.array(r.rms[:3]), o.rad(), rtol=1e-10)
+            assert np.allclose(np.array(r.rms[:ms]), o.rad(), rtol=1e-10)
             maxdiff = np.abs(ref - prev).max()
             assert abs(r.maxdiff - maxdiff) <= 1e-9 * maxdiff
             o.close()
+
+
+def test_batch_fp64_matches_oracle():
+    check_fp64_against_oracle([inst(*s) for s in SWEEP])
+
+
+def test_batch_rectangular_spin1_fp64_matches_oracle():
+    check_fp64_against_oracle([inst_shape(*s) for s in SPIN1_SHAPES])
+
+
+def test_batch_spin2_fp64_matches_oracle():
+    """Spin-2 instances of several shapes, one of them with SOC = 0 and
+    MAG = 0.3 (the Newton projection from its (20, 20) start)."""
+    check_fp64_against_oracle([inst_shape(*s) for s in SPIN2_SHAPES])
+
+
+def test_batch_spin2_fp32_densities():
+    inps = [inst_shape(*s) for s in SPIN2_SHAPES]
+    params = [fgpe.derive(i, FGPE_FP32)[0] for i in inps]
+    steps = 20
+    with fgpe.Batch(params) as b:
+        b.initial_guess()
+        b.run(steps)
+        for i, inp in enumerate(inps):
+            o = oracle_start(inp)
+            o.run(steps)
+            dens = rel_l2(np.abs(b.download(i)) ** 2, np.abs(o.get_phi()) ** 2)
+            print("instance", i, "grid", tuple(params[i].n[:2]), "fp32 densities", dens, flush=True)
+            assert dens < 1e-5
+            o.close()
+
+
+def test_rectangular_spin2_alone_in_batch_and_single_instance_agree():
+    inps = [inst_shape(*s) for s in SPIN2_SHAPES[1:3]]
+    params = [fgpe.derive(i, FGPE_FP64)[0] for i in inps]
+    with fgpe.Batch(params) as b:
+        b.initial_guess()
+        b.run(15)
+        together = [b.download(i) for i in range(len(params))]
+    for i in range(len(params)):
+        with fgpe.Batch([params[i]]) as b1:
+            b1.initial_guess()
+            b1.run(15)
+            alone = b1.download(0)
+        assert rel_l2(alone, together[i]) < 1e-14
+        with fgpe.Solver(params[i]) as s:
+            s.initial_guess()
+            s.normalize()
+            s.run(15)
+            single = s.download()
+        assert rel_l2(single, together[i]) < 1e-12
 
 
 def test_batch_fp32_densities():
