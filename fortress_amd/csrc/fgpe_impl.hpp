@@ -56,17 +56,22 @@ template <> struct Cfg<float> {
   static constexpr int TXK2 = 8;
 };
 template <> struct Cfg<double> {
-  static constexpr int QX = 8;
+  // pass A at 4 points per thread: at 8 the five components' lines held
+  // 256 VGPRs + 161 AGPRs (one wave per SIMD); 256^3 spin-2 1745 -> 924 us
+  static constexpr int QX = 4;
   static constexpr int QY = 8;
   static constexpr int QZ = 8;
-  static constexpr int QK = 4;
+  // pass D at 8 points per thread in blocks of at most 512 threads (256
+  // VGPRs); 4 points per thread made 1024-thread blocks capped at 128 VGPRs
+  // that spilled (256^3 spin-2: 743 -> 630 us)
+  static constexpr int QK = 8;
   static constexpr int Q1 = 8;
   static constexpr int TX = 8;
-  // pass D in 16-column blocks (two 128 B rows; 8 for 512-point lines, where
-  // 16 columns would exceed 1024 threads).  8-column blocks of 96-point lines
-  // (128 threads, each line spread over both waves) gave results that changed
-  // from run to run on grids of 32x96x192 and larger; 16 columns are
-  // bit-reproducible on every grid tried
+  // pass D in 16-column blocks (two 128 B rows; 8 for lines of 384 and 512
+  // points, where 16 columns would exceed 512 threads).  8-column blocks of
+  // 96-point lines (128 threads, each line spread over both waves) gave
+  // results that changed from run to run on grids of 32x96x192 and larger;
+  // 16 columns are bit-reproducible on every grid tried
   static constexpr int TXK = 16;
   // q2D: pass A at 2 points per thread, one line per block; the k pass in
   // tiles of 2 columns (256^2 spin-2: 13.0k -> 21.3k steps/s; 2 points per

@@ -114,9 +114,12 @@ template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t
 // 3D pass D (SOC, inverse y, 1/N) over x in [x0, x0 + xw) on stream st.
 template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
   // columns per block: TXK, fewer where P = N/Q threads per line would make
-  // the block exceed 1024 threads (FP64 512-point lines: 8)
-  constexpr int Q = q_for(N, Cfg<T>::QK), P = N / Q, TX = Cfg<T>::TXK * P <= 1024 ? Cfg<T>::TXK : 1024 / P;
-  static_assert(TX * P <= 1024 && TX >= 8, "pass D block shape");
+  // the block exceed its thread limit (FP64: 512, so the five components'
+  // lines keep 256 VGPRs; FP32: 1024)
+  constexpr int Q = q_for(N, Cfg<T>::QK), P = N / Q;
+  constexpr int MAXT = sizeof(T) == 8 ? 512 : 1024;
+  constexpr int TX = Cfg<T>::TXK * P <= MAXT ? Cfg<T>::TXK : MAXT / P;
+  static_assert(TX * P <= MAXT && TX >= 8, "pass D block shape");
   const size_t lds = dpass_lds_bytes<T, SPIN, N, Q, TX>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "d pass needs %zu B of LDS", lds);
   auto kern = k_dpass<T, SPIN, N, Q, TX>;

@@ -14,7 +14,7 @@ from fortress_amd.abi import FGPE_FP32, FGPE_FP64
 
 pytestmark = pytest.mark.gpu
 
-GRIDS_3D = [(32, 96, 192), (96, 96, 96), (64, 96, 256), (32, 192, 96), (64, 64, 192), (32, 384, 32), (32, 32, 512), (32, 512, 32)]
+GRIDS_3D = [(32, 96, 192), (96, 96, 96), (64, 96, 256), (32, 192, 96), (64, 64, 192), (32, 384, 32), (32, 32, 512), (32, 512, 32), (32, 256, 64)]
 
 
 def twice(inp, prec, nsteps, repeats=3):
