@@ -398,10 +398,12 @@ def surrogate_bench(args):
 
 
 def batch_bench(args):
-    """Batched q2D spin-1 sweep: args.instances imaginary-time instances with
-    seeded grid sizes from {64,128,192,256}^2, DX = 0.1, DT = 0.001, 87Rb spin
-    1 with A0 = 101.8 a_B, A2 - A0 ~ U[-5,5], GAMMAX, GAMMAY ~ U[0,2], NATOMS
-    ~ logU[1e3,1e5], trap 20 Hz in-plane with aspect ~ U[0.8,1.25], FP32.
+    """Batched q2D sweep: args.instances imaginary-time instances with seeded
+    grid sizes from {64,128,192,256}^2 (--batch-rect: each axis drawn on its
+    own), DX = 0.1, DT = 0.001; 87Rb spin 1 with A0 = 101.8 a_B, A2 - A0 ~
+    U[-5,5] (--batch-spin 2: spin 2 with A0, A2, A4 = 87.93, 91.28, 99.18 a_B
+    each scaled by U[0.97,1.03]), GAMMAX, GAMMAY ~ U[0,2], NATOMS ~
+    logU[1e3,1e5], trap 20 Hz in-plane with aspect ~ U[0.8,1.25], FP32.
     After the autotuner, args.steps batched steps are timed; the same steps
     of a sample of the instances one at a time (fgpe_run on a single-instance
     handle each) give the unbatched rate beside it."""
@@ -409,18 +411,27 @@ def batch_bench(args):
     from fortress_amd import fgpe
     from fortress_amd.abi import FGPE_FP32, make_input
     rng = np.random.default_rng(args.seed)
+    spin = args.batch_spin
     inps = []
     for _ in range(args.instances):
-        n = int(rng.choice([64, 128, 192, 256]))
-        a0 = 101.8
-        inps.append(make_input(2, 1, (n, n), (0.1, 0.1), 1e-3, 87.0, (a0, a0 + rng.uniform(-5, 5), 0.0),
+        nx = int(rng.choice([64, 128, 192, 256]))
+        ny = int(rng.choice([64, 128, 192, 256])) if args.batch_rect else nx
+        if spin == 1:
+            a0 = 101.8
+            a = (a0, a0 + rng.uniform(-5, 5), 0.0)
+        else:
+            a = tuple(v * rng.uniform(0.97, 1.03) for v in (87.93, 91.28, 99.18))
+        inps.append(make_input(2, spin, (nx, ny), (0.1, 0.1), 1e-3, 87.0, a,
                                int(round(10 ** rng.uniform(3, 5))), (20.0, 20.0 * rng.uniform(0.8, 1.25), 400.0),
                                (rng.uniform(0, 2), rng.uniform(0, 2)), switch_im=1))
     if args.batch_solve:
+        if args.batch_rect or spin != 1:
+            sys.exit("--batch-solve runs the square spin-1 sweep")
         return batch_solve(args, inps)
     params = [fgpe.derive(i, FGPE_FP32)[0] for i in inps]
     sizes = [p.n[0] for p in params]
-    pts = sum(n * n for n in sizes)
+    pts = sum(p.n[0] * p.n[1] for p in params)
+    ms_comp = 2 * spin + 1
     with fgpe.Batch(params) as b:
         b.initial_guess()
         ms_tune, chosen = b.tune(3)
@@ -428,9 +439,8 @@ def batch_bench(args):
         b.run(args.steps)
         ms = b.last_ms()
     rate = args.instances * args.steps / (ms * 1e-3)
-    # one at a time: the power-of-two instances (the single-instance path has
-    # no 192-point lines), a bounded sample
-    sample = [i for i, n in enumerate(sizes) if n != 192][: args.sample]
+    # one at a time: a bounded sample of the instances
+    sample = list(range(len(params)))[: args.sample]
     t_single = 0.0
     for i in sample:
         with fgpe.Solver(params[i]) as s:
@@ -443,21 +453,22 @@ def batch_bench(args):
             s.sync()
             t_single += time.perf_counter() - t0
     single_rate = len(sample) * args.steps / t_single
-    # HBM bytes per step: two passes, each reading and writing the 3-component FP32 state
-    bytes_step = 2 * 2 * pts * 3 * 8
+    # HBM bytes per step: two passes, each reading and writing the FP32 state
+    bytes_step = 2 * 2 * pts * ms_comp * 8
     line = {
-        "metric": "batched q2D spin-1 sweep: instance-steps/s",
+        "metric": "batched q2D spin-%d sweep: instance-steps/s" % spin,
         "value": round(rate, 1), "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32 propagation, fp64 reductions",
         "data": "synthetic seeded parameter sweep, Gaussian initial guesses",
-        "config": {"workload": "%d q2D spin-1 imaginary-time instances, grids from {64,128,192,256}^2, DX=0.1, "
-                               "DT=0.001, autotuned schedules" % args.instances,
-                   "instances_per_size": {str(n): sizes.count(n) for n in (64, 128, 192, 256)},
+        "config": {"workload": "%d q2D spin-%d imaginary-time instances, grids from {64,128,192,256}%s, DX=0.1, "
+                               "DT=0.001, autotuned schedules"
+                               % (args.instances, spin, " per axis" if args.batch_rect else "^2"),
+                   "instances_per_x_size": {str(n): sizes.count(n) for n in (64, 128, 192, 256)},
                    "tune_ms": [round(x, 4) for x in ms_tune], "chosen": list(chosen)},
         "step_GBps": round(bytes_step * args.steps / (ms * 1e-3) / 1e9, 1),
         "unbatched": {"value": round(single_rate, 1), "unit": "instance-steps/s",
-                      "sample": "%d power-of-two instances, one handle and fgpe_run each" % len(sample)},
+                      "sample": "%d instances, one handle and fgpe_run each" % len(sample)},
         "speedup_vs_unbatched": round(rate / single_rate, 2),
     }
     print(json.dumps(line), flush=True)
@@ -485,10 +496,9 @@ def batch_solve(args, inps):
         sweep_s = time.perf_counter() - t0
         steps = np.array([b.result(i).steps for i in range(len(params))])
     by_size = {n: steps[[i for i, m in enumerate(sizes) if m == n]] for n in (64, 128, 192, 256)}
-    # unbatched FP64 rate per size class (192 has no single-instance path:
-    # its rate is taken as the mean of 128 and 256 per point)
+    # unbatched FP64 rate per size class, one instance of each
     rate = {}
-    for n in (64, 128, 256):
+    for n in (64, 128, 192, 256):
         i = sizes.index(n)
         with fgpe.Solver(params[i]) as s:
             s.initial_guess()
@@ -499,7 +509,6 @@ def batch_solve(args, inps):
             s.run(args.steps)
             s.sync()
             rate[n] = args.steps / (time.perf_counter() - t)
-    rate[192] = 2.0 / (1.0 / rate[128] + 1.0 / rate[256])
     unbatched_s = sum(by_size[n].sum() / rate[n] for n in by_size)
     line = {
         "metric": "batched q2D spin-1 sweep solved to TOL: wall clock",
@@ -583,6 +592,8 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="instead of the 3D propagator, the batched q2D parameter sweep")
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--batch-spin", type=int, default=1, choices=(1, 2), help="with --batch: spin of the sweep")
+    ap.add_argument("--batch-rect", action="store_true", help="with --batch: draw each grid axis on its own")
     ap.add_argument("--sample", type=int, default=16)
     ap.add_argument("--batch-solve", action="store_true",
                     help="with --batch: solve the sweep to --batch-tol in FP64 instead of timing steps")
