@@ -380,6 +380,86 @@ __device__ __forceinline__ void line_fft(C (&v)[Q], C* lds, const R_t<decltype(C
   FftStage<N, Q, 1, DIR, C>::run(v, lds, tw, j);
 }
 
+// The same stages for M lines at once (v[m] the m-th line, its LDS scratch at
+// lds + m * lstride): each exchange of all M lines shares one pair of
+// barriers, where M calls of line_fft take M pairs.  For passes whose blocks
+// hold few lines, where the barriers and LDS round trips, not the
+// arithmetic, set the time.
+template <int N, int Q, int NS, int DIR, int M, typename C> struct FftStageM {
+  static constexpr int R = pick_radix(N / NS, Q);
+  static constexpr int P = N / Q;
+  static constexpr bool LAST = (NS * R == N);
+  static_assert(R > 0, "unsupported FFT size / points-per-thread");
+  using Rw = R_t<decltype(C::x)>;
+  __device__ __forceinline__ static void run(C (&v)[M][Q], C* lds, int lstride, const Rw* tw, int j) {
+    C x[M][Q];
+    SFor<0, M>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      SFor<0, Q / R>::run([&](auto ui) {
+        constexpr int u = decltype(ui)::value;
+        const int b = j + u * P;
+        const int k = b % NS;
+        C y[R];
+        SFor<0, R>::run([&](auto ri) {
+          constexpr int r = decltype(ri)::value;
+          y[r] = v[m][u + r * (Q / R)];
+        });
+        if constexpr (NS > 1) {
+          SFor<1, R>::run([&](auto ri) {
+            constexpr int r = decltype(ri)::value;
+            y[r] = rot<DIR>(y[r], tw[r * NS + k]);
+          });
+        }
+        Dft<R, DIR, C>::run(y);
+        SFor<0, R>::run([&](auto ri) {
+          constexpr int r = decltype(ri)::value;
+          x[m][u + r * (Q / R)] = y[r];
+        });
+      });
+    });
+    if constexpr (LAST) {
+      SFor<0, M>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        SFor<0, Q>::run([&](auto qi) {
+          constexpr int q = decltype(qi)::value;
+          v[m][q] = x[m][q];
+        });
+      });
+    } else {
+      __syncthreads();  // previous readers of these LDS lines are done
+      SFor<0, M>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        C* line = lds + m * lstride;
+        SFor<0, Q / R>::run([&](auto ui) {
+          constexpr int u = decltype(ui)::value;
+          const int b = j + u * P;
+          const int k = b % NS;
+          const int idxd = (b / NS) * NS * R + k;
+          SFor<0, R>::run([&](auto ri) {
+            constexpr int r = decltype(ri)::value;
+            line[lpad(idxd + r * NS)] = x[m][u + r * (Q / R)];
+          });
+        });
+      });
+      __syncthreads();
+      SFor<0, M>::run([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        const C* line = lds + m * lstride;
+        SFor<0, Q>::run([&](auto qi) {
+          constexpr int q = decltype(qi)::value;
+          v[m][q] = line[lpad(j + P * q)];
+        });
+      });
+      FftStageM<N, Q, NS * R, DIR, M, C>::run(v, lds, lstride, tw + (NS > 1 ? NS * R : 0), j);
+    }
+  }
+};
+template <int N, int Q, int DIR, int M, typename C>
+__device__ __forceinline__ void line_fft_multi(C (&v)[M][Q], C* lds, int lstride, const R_t<decltype(C::x)>* tw,
+                                               int j) {
+  FftStageM<N, Q, 1, DIR, M, C>::run(v, lds, lstride, tw, j);
+}
+
 // ---------------------------------------------------------------- physics
 // Real-valued parameters of one run, passed by value to every kernel.
 struct Phys {

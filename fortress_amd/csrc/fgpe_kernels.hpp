@@ -193,8 +193,10 @@ __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ pa
 // the block's NORMC partials.
 // FULL: the pass also applies the 1/N of BFT and the kx factor of KE (2D);
 // in 3D passes D and B apply them (they commute with the y transforms), so the
-// VALU-bound pass A carries less.
-template <typename T, int SPIN, int N, int Q, bool FULL = true>
+// VALU-bound pass A carries less.  MULTI: the MS components' lines are
+// transformed together (line_fft_multi, MS LDS lines per x line), for blocks
+// of one line whose time is the transforms' barriers (q2D).
+template <typename T, int SPIN, int N, int Q, bool FULL = true, bool MULTI = false>
 __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
                                             const Phys& ph, int flags, double* __restrict__ partials) {
   using C = C_t<T>;
@@ -214,7 +216,7 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
   const long nlines = (long)g.ny * g.nz;
   const bool valid = line < nlines;
   const int y = valid ? (int)(line % g.ny) : 0, z = valid ? (int)(line / g.ny) : 0;
-  C* my = lds + tl * LS;
+  C* my = lds + tl * (MULTI ? MS : 1) * LS;
 
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twx[i];
   if (flags & XF_FWD)
@@ -235,7 +237,10 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
   });
   __syncthreads();
   if (flags & XF_INV) {
-    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j); });
+    if constexpr (MULTI)
+      line_fft_multi<N, Q, +1, MS>(v, my, LS, lds_tw, j);
+    else
+      SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j); });
   }
   if (flags & XF_PHYS) {
     const T dt = (T)ph.dt;
@@ -291,27 +296,36 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
   };
   if (flags & XF_FWD) {
     // forward FFT, the kx factor of the next step's KE, store: one component
-    // at a time, so the stores overlap the remaining transforms
-    SFor<0, MS>::run([&](auto mi) {
+    // at a time, so the stores overlap the remaining transforms (all at once
+    // with MULTI)
+    auto kx_store = [&](auto mi) {
       constexpr int m = decltype(mi)::value;
-      line_fft<N, Q, -1>(v[m], my, lds_tw, j);
       if constexpr (FULL)
         SFor<0, Q>::run([&](auto qi) {
           constexpr int q = decltype(qi)::value;
           v[m][q] = rot1(v[m][q], lds_kex[j + P * q]);
         });
       store(mi);
-    });
+    };
+    if constexpr (MULTI) {
+      line_fft_multi<N, Q, -1, MS>(v, my, LS, lds_tw, j);
+      SFor<0, MS>::run(kx_store);
+    } else {
+      SFor<0, MS>::run([&](auto mi) {
+        line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j);
+        kx_store(mi);
+      });
+    }
   } else {
     SFor<0, MS>::run(store);
   }
 }
 // blk0: index of the launch's first block in the whole-grid numbering, so a
 // launch over a z-chunk keeps each block's lines and NORMC partial slot
-template <typename T, int SPIN, int N, int Q, bool FULL>
+template <typename T, int SPIN, int N, int Q, bool FULL, bool MULTI>
 __global__ __launch_bounds__(256) void k_xpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int flags,
                                                double* __restrict__ partials, int blk0) {
-  xpass_block<T, SPIN, N, Q, FULL>(blk0 + (int)blockIdx.x, phi, g, tb, ph, flags, partials);
+  xpass_block<T, SPIN, N, Q, FULL, MULTI>(blk0 + (int)blockIdx.x, phi, g, tb, ph, flags, partials);
 }
 
 // ------------------------------------------------------- pass B/D (y)
@@ -486,13 +500,15 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_dpass(C_t<T>* __restrict__ phi,
 // Lines along y, all components: forward FFT, the ky factor of KE
 // (SRC/cgpe2d.f90:812-829), SOC (SRC/cgpe2d.f90:1554-1650, factors from the
 // FP64-built table soc2), inverse FFT.  A block owns TX consecutive x.
-// LDS: stage twiddles | KE factors [MS][N] | one padded line per column.
-template <typename T, int SPIN, int N, int Q, int TX> constexpr size_t kpass_lds_bytes() {
+// LDS: stage twiddles | KE factors [MS][N] | one padded line per column (MS
+// per column with MULTI: the components' lines transformed together, as in
+// the q2D x pass).
+template <typename T, int SPIN, int N, int Q, int TX, bool MULTI = false> constexpr size_t kpass_lds_bytes() {
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
   return (size_t)TWS * sizeof(R_t<T>) + (size_t)(2 * SPIN + 1) * N * sizeof(R_t<T>) +
-         (size_t)TX * line_stride(N) * sizeof(C_t<T>);
+         (size_t)TX * (MULTI ? 2 * SPIN + 1 : 1) * line_stride(N) * sizeof(C_t<T>);
 }
-template <typename T, int SPIN, int N, int Q, int TX>
+template <typename T, int SPIN, int N, int Q, int TX, bool MULTI = false>
 __device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, const Geom& g, const Tables<T>& tb,
                                             const Phys& ph, int scale) {
   using C = C_t<T>;
@@ -522,8 +538,11 @@ __device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, co
     });
   });
   __syncthreads();
-  C* my = lds + l * LS;
-  SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
+  C* my = lds + l * (MULTI ? MS : 1) * LS;
+  if constexpr (MULTI)
+    line_fft_multi<N, Q, -1, MS>(v, my, LS, lds_tw, j);
+  else
+    SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j); });
   SFor<0, Q>::run([&](auto qi) {
     constexpr int q = decltype(qi)::value;
     const int pos = j + P * q;
@@ -544,16 +563,24 @@ __device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, co
     __builtin_amdgcn_sched_barrier(0);
   });
   // each component is stored as soon as its inverse FFT is done, so the
-  // stores overlap the remaining transforms
-  SFor<0, MS>::run([&](auto mi) {
+  // stores overlap the remaining transforms (all at once with MULTI)
+  auto store = [&](auto mi) {
     constexpr int m = decltype(mi)::value;
-    line_fft<N, Q, +1>(v[m], my, lds_tw, j);
     const CompBuf<C> buf(phi + m * g.npts, g.npts);
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;
       buf.st(v[m][q], voff, q * sstep);
     });
-  });
+  };
+  if constexpr (MULTI) {
+    line_fft_multi<N, Q, +1, MS>(v, my, LS, lds_tw, j);
+    SFor<0, MS>::run(store);
+  } else {
+    SFor<0, MS>::run([&](auto mi) {
+      line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j);
+      store(mi);
+    });
+  }
 }
 // Block coordinates of the all-component passes.  When a block's rows are 64 B (half of a
 // 128 B line), the two blocks sharing each line are given workgroup ids 8
@@ -573,11 +600,11 @@ template <int ROW_BYTES> __device__ __forceinline__ void kpass_coords(int& bx, i
   by = l / nbx;
 }
 
-template <typename T, int SPIN, int N, int Q, int TX>
+template <typename T, int SPIN, int N, int Q, int TX, bool MULTI>
 __global__ __launch_bounds__(TX *(N / Q)) void k_kpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph, int scale) {
   int bx, by;
   kpass_coords<TX * (int)sizeof(C_t<T>)>(bx, by);
-  kpass_block<T, SPIN, N, Q, TX>(bx, phi, g, tb, ph, scale);
+  kpass_block<T, SPIN, N, Q, TX, MULTI>(bx, phi, g, tb, ph, scale);
 }
 
 // ------------------------------------------------------------------- 1D

@@ -27,13 +27,16 @@ int launch_x_q(fgpe_handle* h, int flags, hipStream_t st, int z0, int zc) {
   const long lines0 = (long)z0 * h->ny, lines = (long)zc * h->ny;
   const int blocks = (int)x_pass_blocks<T>(h->dim, N, lines);
   const int blk0 = (int)(lines0 / lpw);
-  const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) + (size_t)lpw * line_stride(N) * sizeof(C<T>);
+  // q2D (FULL) transforms the MS components' lines together: MS LDS lines per x line
+  constexpr bool MULTI = FULL;
+  const size_t lds = (TWS + N) * sizeof(R_t<T>) + 16 * MS * sizeof(double) +
+                     (size_t)lpw * (MULTI ? MS : 1) * line_stride(N) * sizeof(C<T>);
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "x pass needs %zu B of LDS", lds);
   if (flags & XF_PHYS) {
     int rc = ensure_partials(h, (size_t)x_pass_blocks<T>(h->dim, N, (long)h->ny * h->nz) * MS);
     if (rc) return rc;
   }
-  auto kern = k_xpass<T, SPIN, N, Q, FULL>;
+  auto kern = k_xpass<T, SPIN, N, Q, FULL, MULTI>;
   allow_lds(kern, lds);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
@@ -82,9 +85,12 @@ int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw,
 template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
   constexpr int Q = q_for(N, Cfg<T>::QK2), P = N / Q, TX = Cfg<T>::TXK2;
   static_assert(TX * P <= 1024, "2D k pass block shape");
-  const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX>();
+  // the components' lines transformed together where their LDS lines fit in
+  // 128 KB (not at 512 points)
+  constexpr bool MULTI = kpass_lds_bytes<T, SPIN, N, Q, TX, true>() <= 131072;
+  const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX, MULTI>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
-  auto kern = k_kpass<T, SPIN, N, Q, TX>;
+  auto kern = k_kpass<T, SPIN, N, Q, TX, MULTI>;
   allow_lds(kern, lds);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(h->nx / TX), dim3(TX * P), lds, h->stream, (C<T>*)h->phi, geom(h), tables<T>(h),
