@@ -312,11 +312,27 @@ __host__ __device__ constexpr int twiddle_size(int n, int q) {
 __host__ __device__ constexpr int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int line_stride(int n) { return lpad(n) + 1; }
 
+// Barrier between the LDS writes and reads of an exchange.  WAVE: every line
+// lies inside one wavefront (its P threads consecutive, P dividing 64), and
+// the LDS performs one wavefront's operations in issue order, so the
+// exchange only needs the compiler to keep the wave's accesses in program
+// order (a wavefront-scope fence pair around a wave barrier); other waves of
+// the block are not waited for.  Otherwise a block barrier.
+template <bool WAVE> __device__ __forceinline__ void line_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham stage.  Thread j of the line owns butterflies b = j + u*P,
 // u < Q/R; its inputs are always its registers v[u + r*Q/R] (positions
 // j + P*(u + r*Q/R)), so registers hold positions j + P*r' before every stage
 // and after the last one.  Intermediate outputs go through LDS.
-template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
+template <int N, int Q, int NS, int DIR, typename C, bool WAVE> struct FftStage {
   static constexpr int R = pick_radix(N / NS, Q);
   static constexpr int P = N / Q;
   static constexpr bool LAST = (NS * R == N);
@@ -351,7 +367,7 @@ template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
         v[q] = x[q];
       });
     } else {
-      __syncthreads();  // previous readers of this LDS line are done
+      line_sync<WAVE>();  // previous readers of this LDS line are done
       SFor<0, Q / R>::run([&](auto ui) {
         constexpr int u = decltype(ui)::value;
         const int b = j + u * P;
@@ -362,22 +378,23 @@ template <int N, int Q, int NS, int DIR, typename C> struct FftStage {
           lds[lpad(idxd + r * NS)] = x[u + r * (Q / R)];
         });
       });
-      __syncthreads();
+      line_sync<WAVE>();
       SFor<0, Q>::run([&](auto qi) {
         constexpr int q = decltype(qi)::value;
         v[q] = lds[lpad(j + P * q)];
       });
-      FftStage<N, Q, NS * R, DIR, C>::run(v, lds, tw + (NS > 1 ? NS * R : 0), j);
+      FftStage<N, Q, NS * R, DIR, C, WAVE>::run(v, lds, tw + (NS > 1 ? NS * R : 0), j);
     }
   }
 };
 
 // In-place line FFT of N points held as v[q] = x[j + P*q] by the P threads
 // of a line; afterwards v[q] = X[j + P*q].  lds: this line's LDS scratch
-// (line_stride(N) elements); tw: the per-stage twiddle tables.
-template <int N, int Q, int DIR, typename C>
+// (line_stride(N) elements); tw: the per-stage twiddle tables.  WAVE: see
+// line_sync.
+template <int N, int Q, int DIR, bool WAVE = false, typename C>
 __device__ __forceinline__ void line_fft(C (&v)[Q], C* lds, const R_t<decltype(C::x)>* tw, int j) {
-  FftStage<N, Q, 1, DIR, C>::run(v, lds, tw, j);
+  FftStage<N, Q, 1, DIR, C, WAVE>::run(v, lds, tw, j);
 }
 
 // The same stages for M lines at once (v[m] the m-th line, its LDS scratch at
@@ -385,7 +402,7 @@ __device__ __forceinline__ void line_fft(C (&v)[Q], C* lds, const R_t<decltype(C
 // barriers, where M calls of line_fft take M pairs.  For passes whose blocks
 // hold few lines, where the barriers and LDS round trips, not the
 // arithmetic, set the time.
-template <int N, int Q, int NS, int DIR, int M, typename C> struct FftStageM {
+template <int N, int Q, int NS, int DIR, int M, typename C, bool WAVE> struct FftStageM {
   static constexpr int R = pick_radix(N / NS, Q);
   static constexpr int P = N / Q;
   static constexpr bool LAST = (NS * R == N);
@@ -426,7 +443,7 @@ template <int N, int Q, int NS, int DIR, int M, typename C> struct FftStageM {
         });
       });
     } else {
-      __syncthreads();  // previous readers of these LDS lines are done
+      line_sync<WAVE>();  // previous readers of these LDS lines are done
       SFor<0, M>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
         C* line = lds + m * lstride;
@@ -441,7 +458,7 @@ template <int N, int Q, int NS, int DIR, int M, typename C> struct FftStageM {
           });
         });
       });
-      __syncthreads();
+      line_sync<WAVE>();
       SFor<0, M>::run([&](auto mi) {
         constexpr int m = decltype(mi)::value;
         const C* line = lds + m * lstride;
@@ -450,14 +467,14 @@ template <int N, int Q, int NS, int DIR, int M, typename C> struct FftStageM {
           v[m][q] = line[lpad(j + P * q)];
         });
       });
-      FftStageM<N, Q, NS * R, DIR, M, C>::run(v, lds, lstride, tw + (NS > 1 ? NS * R : 0), j);
+      FftStageM<N, Q, NS * R, DIR, M, C, WAVE>::run(v, lds, lstride, tw + (NS > 1 ? NS * R : 0), j);
     }
   }
 };
-template <int N, int Q, int DIR, int M, typename C>
+template <int N, int Q, int DIR, int M, bool WAVE = false, typename C>
 __device__ __forceinline__ void line_fft_multi(C (&v)[M][Q], C* lds, int lstride, const R_t<decltype(C::x)>* tw,
                                                int j) {
-  FftStageM<N, Q, 1, DIR, M, C>::run(v, lds, lstride, tw, j);
+  FftStageM<N, Q, 1, DIR, M, C, WAVE>::run(v, lds, lstride, tw, j);
 }
 
 // ---------------------------------------------------------------- physics

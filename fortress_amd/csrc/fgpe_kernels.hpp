@@ -210,6 +210,8 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
   double* red = reinterpret_cast<double*>(lds_kex + N);
   C* lds = reinterpret_cast<C*>(red + 16 * MS);
 
+  // a line's P threads are consecutive: inside one wavefront when P divides 64
+  constexpr bool WAVE = 64 % P == 0;
   const int lpw = blockDim.x / P;
   const int tl = threadIdx.x / P, j = threadIdx.x % P;
   const long line = (long)blk * lpw + tl;
@@ -238,9 +240,9 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
   __syncthreads();
   if (flags & XF_INV) {
     if constexpr (MULTI)
-      line_fft_multi<N, Q, +1, MS>(v, my, LS, lds_tw, j);
+      line_fft_multi<N, Q, +1, MS, WAVE>(v, my, LS, lds_tw, j);
     else
-      SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1>(v[decltype(mi)::value], my, lds_tw, j); });
+      SFor<0, MS>::run([&](auto mi) { line_fft<N, Q, +1, WAVE>(v[decltype(mi)::value], my, lds_tw, j); });
   }
   if (flags & XF_PHYS) {
     const T dt = (T)ph.dt;
@@ -308,11 +310,11 @@ __device__ __forceinline__ void xpass_block(int blk, C_t<T>* __restrict__ phi, c
       store(mi);
     };
     if constexpr (MULTI) {
-      line_fft_multi<N, Q, -1, MS>(v, my, LS, lds_tw, j);
+      line_fft_multi<N, Q, -1, MS, WAVE>(v, my, LS, lds_tw, j);
       SFor<0, MS>::run(kx_store);
     } else {
       SFor<0, MS>::run([&](auto mi) {
-        line_fft<N, Q, -1>(v[decltype(mi)::value], my, lds_tw, j);
+        line_fft<N, Q, -1, WAVE>(v[decltype(mi)::value], my, lds_tw, j);
         kx_store(mi);
       });
     }
