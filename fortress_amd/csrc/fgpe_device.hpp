@@ -984,4 +984,18 @@ __host__ __device__ __forceinline__ void soc_from_table(const C* t, int imag, C&
   }
 }
 
+// soc_from_table for wave-number indices (x, y) of an nx x ny grid, from the
+// quarter table of soc2_table: the entry of (|kx|, |ky|) (FFT index i and
+// n - i hold opposite wave numbers; i <= n/2 is the non-negative one), with
+// the real part of e1 negated for kx < 0 and the imaginary part for ky < 0.
+template <int SPIN, typename C>
+__host__ __device__ __forceinline__ void soc_from_quarter(const C* t, int x, int y, int nx, int ny, int imag, C& e1,
+                                                          C* dm1) {
+  const bool nxneg = x > nx / 2, nyneg = y > ny / 2;
+  const int xq = nxneg ? nx - x : x, yq = nyneg ? ny - y : y;
+  soc_from_table<SPIN, C>(t + ((long)yq * (nx / 2 + 1) + xq) * (SPIN + 1), imag, e1, dm1);
+  if (nxneg) e1.x = -e1.x;
+  if (nyneg) e1.y = -e1.y;
+}
+
 }  // namespace fgpe

@@ -197,14 +197,18 @@ template <typename T> R_t<T> ke_factor(double a, double dt, int imag) {
   return rot_from<T>(-2.0 * sh * sh, -std::sin(dt * a));
 }
 
-// 2D SOC factors per (ky, kx) for soc_from_table, in FP64 then rounded:
-// ALPHA, BETA and the factors of SRC/cgpe2d.f90 SOC (the 2D form of
-// SRC/cgpe3d.f90:904-1004): e1 = exp(-i BETA) = (GX kx - i GY ky)/|.| (1 when
-// ALPHA <= 1e-30), th = DT L ALPHA with ALPHA = 2 |(GX kx, GY ky)|.
+// 2D SOC factors for soc_from_quarter, in FP64 then rounded: ALPHA, BETA and
+// the factors of SRC/cgpe2d.f90 SOC (the 2D form of SRC/cgpe3d.f90:904-1004):
+// e1 = exp(-i BETA) = (GX kx - i GY ky)/|.| (1 when ALPHA <= 1e-30),
+// th = DT L ALPHA with ALPHA = 2 |(GX kx, GY ky)|.  Only the quarter kx, ky >= 0
+// is stored ([NY/2 + 1][NX/2 + 1][1 + SPIN]; indices 0..N/2 of the FFT-order
+// wave numbers, N/2 the positive Nyquist): ALPHA depends on |kx|, |ky| alone,
+// and negating kx or ky negates the real or imaginary part of e1, which the
+// reader applies (exactly: rounding is symmetric in sign).
 template <typename T>
 std::vector<fgpe::C_t<T>> soc2_table(const std::vector<double>& kx, const std::vector<double>& ky, double gx, double gy,
                              double dt, int imag, int spin) {
-  const size_t nx = kx.size(), ny = ky.size();
+  const size_t nx = kx.size() / 2 + 1, ny = ky.size() / 2 + 1;
   std::vector<fgpe::C_t<T>> t(nx * ny * (spin + 1));
   auto put = [&](size_t i, double a, double b) {
     t[i].x = (T)a;

@@ -44,7 +44,7 @@ template <typename T> struct Tables {
   const double* wy;
   const double* wz;
   const C_t<T>* cscale;  // pending per-component renormalisation [MS]
-  const C_t<T>* soc2;    // 2D SOC factors per (ky, kx): [NY][NX][1 + SPIN] (soc2_table)
+  const C_t<T>* soc2;    // 2D SOC factors of kx, ky >= 0: [NY/2+1][NX/2+1][1 + SPIN] (soc2_table)
 };
 
 struct Geom {
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_dpass(C_t<T>* __restrict__ phi,
     constexpr int q = decltype(qi)::value;
     const int ky = j + P * q;
     C e1, dm1[MS], pt[MS], cr[MS];
-    soc_from_table<SPIN, C>(tb.soc2 + ((long)ky * g.nx + x) * (SPIN + 1), ph.imag, e1, dm1);
+    soc_from_quarter<SPIN, C>(tb.soc2, x, ky, g.nx, g.ny, ph.imag, e1, dm1);
     SFor<0, MS>::run([&](auto mi) { pt[decltype(mi)::value] = v[decltype(mi)::value][q]; });
     soc_corr<SPIN, C, T>(pt, e1, dm1, cr);
     SFor<0, MS>::run([&](auto mi) {
@@ -549,7 +549,7 @@ __device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, co
     constexpr int q = decltype(qi)::value;
     const int pos = j + P * q;
     C e1, dm1[MS];
-    soc_from_table<SPIN, C>(tb.soc2 + ((long)pos * g.nx + x) * (SPIN + 1), ph.imag, e1, dm1);
+    soc_from_quarter<SPIN, C>(tb.soc2, x, pos, g.nx, g.ny, ph.imag, e1, dm1);
     // line-axis KE factor, then the SOC change of the result
     C pt[MS], cr[MS];
     SFor<0, MS>::run([&](auto mi) {
