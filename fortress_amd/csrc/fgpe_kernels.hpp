@@ -34,7 +34,8 @@ template <typename T> struct Tables {
   const R_t<T>* kex;  // exp(-i CDT kx^2) - 1                          [NX]
   const R_t<T>* key;  // exp(-i CDT ky^2) - 1 (3D)                     [NY or 1]
   const R_t<T>* kel;  // line axis: 3D exp(-i CDT (kz^2 - 2 L GAMMAZ kz)) - 1,
-                      // 2D exp(-i CDT ky^2) - 1, per component      [MS][N_line]
+                      // 2D exp(-i CDT ky^2) - 1 (the same in every row;
+                      // the 2D k pass reads row 0), per component  [MS][N_line]
   const T* kx;        // wave numbers (FFT order, positive Nyquist)
   const T* ky;
   const T* vx;  // ALPHAX^2 x^2
@@ -502,12 +503,13 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_dpass(C_t<T>* __restrict__ phi,
 // Lines along y, all components: forward FFT, the ky factor of KE
 // (SRC/cgpe2d.f90:812-829), SOC (SRC/cgpe2d.f90:1554-1650, factors from the
 // FP64-built table soc2), inverse FFT.  A block owns TX consecutive x.
-// LDS: stage twiddles | KE factors [MS][N] | one padded line per column (MS
-// per column with MULTI: the components' lines transformed together, as in
-// the q2D x pass).
+// LDS: stage twiddles | KE factors [N] (in 2D the ky^2 factor is the same
+// for every component: row 0 of the per-component table) | one padded line
+// per column (MS per column with MULTI: the components' lines transformed
+// together, as in the q2D x pass).
 template <typename T, int SPIN, int N, int Q, int TX, bool MULTI = false> constexpr size_t kpass_lds_bytes() {
   constexpr int TWS = twiddle_size(N, Q) > 0 ? twiddle_size(N, Q) : 1;
-  return (size_t)TWS * sizeof(R_t<T>) + (size_t)(2 * SPIN + 1) * N * sizeof(R_t<T>) +
+  return (size_t)TWS * sizeof(R_t<T>) + (size_t)N * sizeof(R_t<T>) +
          (size_t)TX * (MULTI ? 2 * SPIN + 1 : 1) * line_stride(N) * sizeof(C_t<T>);
 }
 template <typename T, int SPIN, int N, int Q, int TX, bool MULTI = false>
@@ -521,11 +523,11 @@ __device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, co
   extern __shared__ __align__(16) unsigned char smem_raw[];
   R_t<T>* lds_tw = reinterpret_cast<R_t<T>*>(smem_raw);
   R_t<T>* lds_ke = lds_tw + TWS;
-  C* lds = reinterpret_cast<C*>(lds_ke + MS * N);
+  C* lds = reinterpret_cast<C*>(lds_ke + N);
   const int l = threadIdx.x % TX, j = threadIdx.x / TX;
   const int x = bx * TX + l;
   for (int i = threadIdx.x; i < TWS; i += blockDim.x) lds_tw[i] = tb.twy[i];
-  for (int i = threadIdx.x; i < MS * N; i += blockDim.x) lds_ke[i] = tb.kel[i];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) lds_ke[i] = tb.kel[i];
   const unsigned int voff = (unsigned int)((x + (long)j * g.nx) * (long)sizeof(C));
   const unsigned int sstep = (unsigned int)((long)P * g.nx * (long)sizeof(C));
   C v[MS][Q];
@@ -552,9 +554,10 @@ __device__ __forceinline__ void kpass_block(int bx, C_t<T>* __restrict__ phi, co
     soc_from_quarter<SPIN, C>(tb.soc2, x, pos, g.nx, g.ny, ph.imag, e1, dm1);
     // line-axis KE factor, then the SOC change of the result
     C pt[MS], cr[MS];
+    const R_t<T> ke = lds_ke[pos];
     SFor<0, MS>::run([&](auto mi) {
       constexpr int m = decltype(mi)::value;
-      v[m][q] = rot1(v[m][q], lds_ke[m * N + pos]);
+      v[m][q] = rot1(v[m][q], ke);
       pt[m] = v[m][q];
     });
     soc_corr<SPIN, C, T>(pt, e1, dm1, cr);
