@@ -168,8 +168,8 @@ constexpr int NGRP = NCLS * NCLS;  // shape groups: (x-length class, y-length cl
 constexpr int CLS_N[NCLS] = {64, 128, 192, 256};
 // schedule s = 2*length class + variant: points per thread of the x pass and
 // of the k pass (FP64 holds twice the registers per point: smaller Q)
-constexpr int SQX32[FGPE_BATCH_NSCHED] = {4, 8, 4, 8, 12, 6, 4, 8};
-constexpr int SQK32[FGPE_BATCH_NSCHED] = {8, 4, 8, 16, 12, 6, 8, 16};
+constexpr int SQX32[FGPE_BATCH_NSCHED] = {4, 8, 4, 4, 12, 6, 4, 4};
+constexpr int SQK32[FGPE_BATCH_NSCHED] = {8, 4, 8, 4, 12, 6, 8, 4};
 constexpr int SQX64[FGPE_BATCH_NSCHED] = {8, 4, 8, 4, 6, 12, 8, 4};
 constexpr int SQK64[FGPE_BATCH_NSCHED] = {4, 8, 4, 8, 6, 6, 4, 8};
 constexpr int BTX = 8;  // x columns per k-pass block

@@ -310,7 +310,7 @@ typedef struct fgpe_batch_result {
 /* Schedules (points per thread of the x pass / the k pass), two per line
  * length: s = 2*c + v for c = 0..3 (64, 128, 192, 256).  An instance's x pass
  * follows the schedule of its x length, its k pass that of its y length.  FP32: 64 (4/8, 8/4),
- * 128 (4/8, 8/16), 192 (12/12, 6/6), 256 (4/8, 8/16); FP64: 64 (8/4, 4/8),
+ * 128 (4/8, 4/4), 192 (12/12, 6/6), 256 (4/8, 4/4); FP64: 64 (8/4, 4/8),
  * 128 (8/4, 4/8), 192 (6/6, 12/6), 256 (8/4, 4/8). */
 #define FGPE_BATCH_NSCHED 8
 
