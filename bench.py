@@ -397,6 +397,45 @@ def surrogate_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def surrogate_ceiling(args):
+    """What a coarse-grid guess can gain at best: for args.surrogate_ceiling
+    held-out-style draws, the exact coarse ground state (64^2 and 128^2 over
+    the same box, FP64, TOL args.tol) spectrally interpolated onto the
+    production grid, against the Gaussian guess; steps to TOL of the
+    production solve, same batched loop for all."""
+    from fortress_amd import fgpe
+    from fortress_amd import surrogate as S
+    rng = np.random.default_rng(args.seed + 1)
+    draws = [S.draw(rng) for _ in range(args.surrogate_ceiling)]
+
+    def solve(params, guesses):
+        with fgpe.Batch(params) as b:
+            if guesses is None:
+                b.initial_guess()
+            else:
+                for i, g in enumerate(guesses):
+                    b.upload(i, g)
+                b.normalize()
+            S.solve_batch(b, 100, args.tol, args.max_steps)
+            return [b.result(i) for i in range(len(params))], [b.download(i) for i in range(len(params))]
+
+    fine = [S.problem(d, args.n) for d in draws]
+    rp, _ = solve(fine, None)
+    sp = np.array([r.steps for r in rp], float)
+    out = {"metric": "steps to TOL from the exact coarse ground state, plain / coarse-guess", "draws": len(draws),
+           "config": {"workload": "q2D spin-1 imaginary time, %d^2 over a 25.6 box, DT=0.001, TOL=%g, STP=100, FP64; "
+                                  "coarse states solved to the same TOL" % (args.n, args.tol)}}
+    for m in (64, 128):
+        _, pc = solve([S.problem(d, m) for d in draws], None)
+        rg, _ = solve(fine, [S.upsample(p, args.n) for p in pc])
+        sg = np.array([r.steps for r in rg], float)
+        out["from_%d" % m] = {"total_ratio": round(float(sp.sum() / sg.sum()), 2),
+                              "median_ratio": round(float(np.median(sp / sg)), 1),
+                              "max_energy_rel_gap": float(max(abs(a.en / b.en - 1) for a, b in zip(rg, rp)))}
+        print(json.dumps({"coarse": m, **out["from_%d" % m]}), flush=True)
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args):
     """Batched q2D sweep: args.instances imaginary-time instances with seeded
     grid sizes from {64,128,192,256}^2 (--batch-rect: each axis drawn on its
@@ -581,7 +620,9 @@ def main():
                     help="with --surrogate: evaluate a saved network instead of generating data and training")
     ap.add_argument("--surrogate-train-only", action="store_true",
                     help="with --surrogate: stop after training (and saving)")
-    ap.add_argument("--train-tol", type=float, default=1e-6)
+    ap.add_argument("--train-tol", type=float, default=1e-9)
+    ap.add_argument("--surrogate-ceiling", type=int, default=0,
+                    help="instead of the 3D propagator, the gain of exact coarse ground states as guesses (draws)")
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--epochs", type=int, default=1200)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -618,6 +659,8 @@ def main():
         return batch_bench(args)
     if args.surrogate:
         return surrogate_bench(args)
+    if args.surrogate_ceiling:
+        return surrogate_ceiling(args)
     if args.config1:
         return config1_bench(args)
     if args.config2:
