@@ -409,7 +409,7 @@ int xlaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int ny, int flags
                      nb, ny, flags);
   HIPCHK(hipGetLastError());
   if ((flags & XF_PHYS) && (flags & XF_NORM)) {
-    hipLaunchKernelGGL((kg_finalize<T, MS>), dim3(cnt), dim3(1024), 0, st, (const GInst<T>*)b->ginst,
+    hipLaunchKernelGGL((kg_finalize<T, MS>), dim3(cnt), dim3(fin_threads(nb)), 0, st, (const GInst<T>*)b->ginst,
                        b->dlist + start, nb);
     HIPCHK(hipGetLastError());
   }

@@ -103,6 +103,13 @@ template <typename T> constexpr int x_pass_q(int dim, int nx) {
 template <typename T> constexpr int long_line_q(int n, int qdef) { return q_for(n, (n >= 512 && qdef > 8) ? 8 : qdef); }
 template <typename T> constexpr int y_pass_q(int ny) { return long_line_q<T>(ny, Cfg<T>::QY); }
 template <typename T> constexpr int z_pass_q(int nz) { return long_line_q<T>(nz, Cfg<T>::QZ); }
+// Threads of the one-block NORMC finalisation over nparts partials: about 16
+// per thread, in whole waves, at most 1024 (a q2D grid's few hundred partials
+// then take one wave, which launches and retires sooner than sixteen).
+inline int fin_threads(long nparts) {
+  const long t = ((nparts + 16 * 64 - 1) / (16 * 64)) * 64;
+  return (int)std::min<long>(1024, std::max<long>(64, t));
+}
 // Blocks of one pass-A launch: P = NX/Q threads per x line, lines packed into
 // blocks of at least 256 threads (3D) or of one wave (q2D, where the grid has
 // few lines).  Each block writes MS NORMC partials.

@@ -157,7 +157,7 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
   __shared__ double red[16 * MS];
   double a[MS];
   SFor<0, MS>::run([&](auto mi) { a[decltype(mi)::value] = 0.0; });
-#pragma unroll 4
+#pragma unroll 8
   for (int i = threadIdx.x; i < nparts; i += blockDim.x)
     SFor<0, MS>::run([&](auto mi) {
       constexpr int m = decltype(mi)::value;

@@ -55,7 +55,7 @@ template <typename T, int SPIN, int N> int launch_fin(fgpe_handle* h) {
   constexpr int MS = 2 * SPIN + 1;
   const int blocks = (int)x_pass_blocks<T>(h->dim, N, (long)h->ny * h->nz);
   prof_begin(h);
-  hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(1024), 0, h->stream, (const double*)h->partials, blocks,
+  hipLaunchKernelGGL((k_finalize<T, MS>), dim3(1), dim3(fin_threads(blocks)), 0, h->stream, (const double*)h->partials, blocks,
                      finargs(h), h->dnorm, (C<T>*)h->cscale, h->cscale64, h->dsig, h->derr);
   HIPCHK(hipGetLastError());
   prof_end(h, FGPE_PASS_FIN);
