@@ -257,7 +257,7 @@ struct fgpe_handle {
   // created with the handle
   int zchunk = 0, xchunk = 0, nstreams = 1;
   hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t evs[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};  // fork/join of the 3D schedule; [3]: NORMC finalised
   int dev = 0;
   size_t lds_max = 65536;
   // state

@@ -444,7 +444,7 @@ template <typename T, int SPIN, int N, int Q, int TX> constexpr size_t dpass_lds
   return (size_t)TWS * sizeof(R_t<T>) + (size_t)TX * line_stride(N) * sizeof(C_t<T>);
 }
 template <int ROW_BYTES> __device__ __forceinline__ void kpass_coords(int& bx, int& by);
-template <typename T, int SPIN, int N, int Q, int TX>
+template <typename T, int SPIN, int N, int Q, int TX, bool SCALE>
 __global__ __launch_bounds__(TX *(N / Q)) void k_dpass(C_t<T>* __restrict__ phi, Geom g, Tables<T> tb, Phys ph,
                                                        double inv_npts, int bx0, int z0) {
   using C = C_t<T>;
@@ -469,7 +469,11 @@ __global__ __launch_bounds__(TX *(N / Q)) void k_dpass(C_t<T>* __restrict__ phi,
     const CompBuf<C> buf(phi + m * g.npts, g.npts);
     SFor<0, Q>::run([&](auto qi) {
       constexpr int q = decltype(qi)::value;
-      v[m][q] = buf.ld(voff, q * sstep);
+      const C a = buf.ld(voff, q * sstep);
+      if constexpr (SCALE)
+        v[m][q] = cmul(a, tb.cscale[m]);
+      else
+        v[m][q] = a;
     });
   });
   SFor<0, Q>::run([&](auto qi) {

@@ -118,7 +118,7 @@ template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t
 }
 
 // 3D pass D (SOC, inverse y, 1/N) over x in [x0, x0 + xw) on stream st.
-template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
+template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw, int scale) {
   // columns per block: TXK, fewer where P = N/Q threads per line would make
   // the block exceed its thread limit (FP64: 512, so the five components'
   // lines keep 256 VGPRs; FP32: 1024)
@@ -128,7 +128,9 @@ template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t 
   static_assert(TX * P <= MAXT && TX >= 8, "pass D block shape");
   const size_t lds = dpass_lds_bytes<T, SPIN, N, Q, TX>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "d pass needs %zu B of LDS", lds);
-  auto kern = k_dpass<T, SPIN, N, Q, TX>;
+  // scale: multiply by the pending renormalisation on load (a separate
+  // instantiation, so the kernel without it keeps its registers)
+  auto kern = scale ? k_dpass<T, SPIN, N, Q, TX, true> : k_dpass<T, SPIN, N, Q, TX, false>;
   allow_lds(kern, lds);
   prof_begin(h);
   hipLaunchKernelGGL(kern, dim3(xw / TX, h->nz), dim3(TX * P), lds, st, (C<T>*)h->phi, geom(h), tables<T>(h), phys(h),
@@ -179,7 +181,7 @@ int launch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk) {
   template int fgpe_impl::launch_x<T, SPIN, N>(fgpe_handle*, int, hipStream_t, int, int);              \
   template int fgpe_impl::launch_fin<T, SPIN, N>(fgpe_handle*);                                        \
   template int fgpe_impl::launch_k<T, SPIN, N>(fgpe_handle*, int);                                     \
-  template int fgpe_impl::launch_d<T, SPIN, N>(fgpe_handle*, hipStream_t, int, int);                   \
+  template int fgpe_impl::launch_d<T, SPIN, N>(fgpe_handle*, hipStream_t, int, int, int);              \
   template int fgpe_impl::launch_obs_axis<T, SPIN, 2, 0, N>(fgpe_handle*, const ObsArgs&, size_t, int*); \
   template int fgpe_impl::launch_obs_axis<T, SPIN, 2, 1, N>(fgpe_handle*, const ObsArgs&, size_t, int*); \
   template int fgpe_impl::launch_obs_axis<T, SPIN, 3, 0, N>(fgpe_handle*, const ObsArgs&, size_t, int*); \

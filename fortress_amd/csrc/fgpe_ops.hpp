@@ -15,7 +15,7 @@ template <typename T, int N>
 int launch_y(fgpe_handle* h, int dir, int scale, hipStream_t st, int x0, int xw, int z0, int zc);
 template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale);
 template <typename T, int N> int launch_z(fgpe_handle* h, int scale, hipStream_t st, int x0, int xw);
-template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw);
+template <typename T, int SPIN, int N> int launch_d(fgpe_handle* h, hipStream_t st, int x0, int xw, int scale);
 template <typename T, int SPIN, int N> int launch_1d(fgpe_handle* h, long nsteps);
 template <typename T, int SPIN, int DIM, int AXIS, int N>
 int launch_obs_axis(fgpe_handle* h, const ObsArgs& a, size_t off, int* nblk);
@@ -70,10 +70,10 @@ template <typename T> int dispatch_z(fgpe_handle* h, int scale, hipStream_t st, 
   }
   return fail(FGPE_EINVAL, "unsupported NZ=%d", h->nz);
 }
-template <typename T, int SPIN> int dispatch_d(fgpe_handle* h, hipStream_t st, int x0, int xw) {
+template <typename T, int SPIN> int dispatch_d(fgpe_handle* h, hipStream_t st, int x0, int xw, int scale) {
   switch (h->ny) {
 #define FGPE_CASE(n) \
-  case n: return launch_d<T, SPIN, n>(h, st, x0, xw);
+  case n: return launch_d<T, SPIN, n>(h, st, x0, xw, scale);
     FGPE_LINE_SIZES(FGPE_CASE)
 #undef FGPE_CASE
   }
@@ -94,9 +94,9 @@ template <typename T, int SPIN> int dispatch_1d(fgpe_handle* h, long nsteps) {
 //     next step's B (forward-y, kx and ky factors of KE) -- both act within
 //     z-planes;
 //   phase 2, by x-chunks: C (forward-z, kz/GAMMAZ factor of KE, inverse-z,
-//     one component per block), applying the pending renormalisation, then D
-//     (SOC in (kx, ky, z) space, inverse-y, 1/N, all components) -- both act
-//     within x-slabs.
+//     one component per block), then D (SOC in (kx, ky, z) space,
+//     inverse-y, 1/N, all components), one of them applying the pending
+//     renormalisation -- both act within x-slabs.
 // The SO coupling depends on (kx, ky) only (SRC/cgpe3d.f90:904-1004), so it
 // commutes with the z transforms and is applied after the inverse z FFT,
 // still after the whole KE factor as in the reference's order: pass C then
@@ -105,8 +105,10 @@ template <typename T, int SPIN> int dispatch_1d(fgpe_handle* h, long nsteps) {
 // the memory-bound B of neighbouring chunks, C the heavier D; the phases are
 // joined on the handle's stream (C needs every z of B, A every x of D).  The
 // renormalisation (uniform with SO coupling, per component without it, where
-// no pass mixes components) commutes with the FFTs, KE and SOC, so C applies
-// it once NORMC of every chunk of A is in.  Every chunking gives
+// no pass mixes components) commutes with the FFTs, KE and SOC, so in FP32 D
+// applies it and the NORMC finalisation that sets it runs beside the first C
+// launches instead of between the phases (FP64: C applies it, after the
+// finalisation).  Every chunking gives
 // bit-identical results: each line is transformed by the same code and each
 // pass-A block keeps its NORMC partial slot.
 template <typename T, int SPIN> int run_steps_3d(fgpe_handle* h, long nsteps) {
@@ -135,24 +137,47 @@ template <typename T, int SPIN> int run_steps_3d(fgpe_handle* h, long nsteps) {
     }
     return join();
   };
-  auto phase2 = [&]() -> int {
+  // FP32: D applies the renormalisation, and fin (the NORMC finalisation of
+  // the last pass A) runs first on the handle's stream, beside the other
+  // streams' first C launches (C does not need the scale); those streams wait
+  // for it before their first D.  FP64: C applies it after the finalisation
+  // (D's extra FP64 products cost more than the overlap saves: 256^3 spin-2
+  // imaginary time 2490 -> 2526 us per step with D applying it; FP32
+  // 1221 -> 1194).
+  constexpr bool LATE = sizeof(T) == 4;
+  auto phase2 = [&](bool fin) -> int {
     int rc;
     if ((rc = fork())) return rc;
-    for (int x0 = 0, c = 0; x0 < h->nx; x0 += xc, ++c) {
-      if ((rc = dispatch_z<T>(h, 1, st[c % nst], x0, xc))) return rc;
-      if ((rc = dispatch_d<T, SPIN>(h, st[c % nst], x0, xc))) return rc;
+    if (fin) {
+      if ((rc = dispatch_fin<T, SPIN>(h))) return rc;
+      if (nst > 1) HIPCHK(hipEventRecord(h->evs[3], h->stream));
     }
+    bool waits[3] = {false, fin && nst > 1, fin && nst > 2};
+    for (int x0 = 0, c = 0; x0 < h->nx; x0 += xc, ++c) {
+      const int i = c % nst;
+      if ((rc = dispatch_z<T>(h, LATE ? 0 : 1, st[i], x0, xc))) return rc;
+      if (waits[i]) {
+        HIPCHK(hipStreamWaitEvent(st[i], h->evs[3], 0));
+        waits[i] = false;
+      }
+      if ((rc = dispatch_d<T, SPIN>(h, st[i], x0, xc, LATE ? 1 : 0))) return rc;
+    }
+    for (int i = 1; i < nst; ++i)  // a stream with no chunk still orders after the finalisation
+      if (waits[i]) HIPCHK(hipStreamWaitEvent(st[i], h->evs[3], 0));
     return join();
   };
   int rc;
   if ((rc = phase1(XF_FWD, true))) return rc;  // the run's first forward-x and forward-y
+  bool fin = false;
   for (long s = 1; s <= nsteps; ++s) {
-    if ((rc = phase2())) return rc;
+    if ((rc = phase2(fin))) return rc;
     // NORMC every step in imaginary time (it sets the renormalisation); in
     // real time the scale is 1 and only the last step's norms are reported
     const int flags = XF_INV | XF_PHYS | (s < nsteps ? XF_FWD : 0) | ((h->p.imag || s == nsteps) ? XF_NORM : 0);
     if ((rc = phase1(flags, s < nsteps))) return rc;
-    if ((flags & XF_NORM) && (rc = dispatch_fin<T, SPIN>(h))) return rc;
+    // finalised at the start of the next phase 2 (FP32), or here
+    fin = LATE && (flags & XF_NORM) && s < nsteps;
+    if ((flags & XF_NORM) && !fin && (rc = dispatch_fin<T, SPIN>(h))) return rc;
   }
   return 0;
 }
