@@ -50,7 +50,10 @@ template <> struct Cfg<float> {
   // q2D: a grid has only NY x lines, so pass A runs one line per block to
   // spread over the CUs, at 4 points per thread (radix-4 stages: half the
   // roundings of radix 2); the k pass runs 8 points per thread in tiles of 8
-  // columns (narrower tiles were slower in FP32)
+  // columns.  4 points in tiles of 4 ran the 256^2 spin-2 step in 34.4 us
+  // instead of 39.4, but its fourth radix stage's roundings took the FP32
+  // density error of a q2D spin-2 imaginary-time case over its 1e-5 bar
+  // (1.05e-5)
   static constexpr int QX2 = 4;
   static constexpr int QK2 = 8;
   static constexpr int TXK2 = 8;
@@ -73,12 +76,14 @@ template <> struct Cfg<double> {
   // results that changed from run to run on grids of 32x96x192 and larger;
   // 16 columns are bit-reproducible on every grid tried
   static constexpr int TXK = 16;
-  // q2D: pass A at 2 points per thread, one line per block; the k pass in
-  // tiles of 2 columns (256^2 spin-2: 13.0k -> 21.3k steps/s; 2 points per
+  // q2D: pass A at 2 points per thread, one line per block; the k pass at 4
+  // points per thread in tiles of 4 columns (256^2 spin-2: 13.0k -> 21.3k
+  // steps/s with tiles of 2, then 38.7 -> 36.9 us per step with tiles of 4
+  // once the components' lines were transformed together; 2 points per
   // thread in the k pass was slower)
   static constexpr int QX2 = 2;
   static constexpr int QK2 = 4;
-  static constexpr int TXK2 = 2;
+  static constexpr int TXK2 = 4;
 };
 
 // Line lengths with compiled passes: 2D/3D axes, and the q1D line.  A

@@ -86,7 +86,7 @@ template <typename T, int SPIN, int N> int launch_k(fgpe_handle* h, int scale) {
   constexpr int Q = q_for(N, Cfg<T>::QK2), P = N / Q, TX = Cfg<T>::TXK2;
   static_assert(TX * P <= 1024, "2D k pass block shape");
   // the components' lines transformed together where their LDS lines fit in
-  // 128 KB (not at 512 points)
+  // 128 KB
   constexpr bool MULTI = kpass_lds_bytes<T, SPIN, N, Q, TX, true>() <= 131072;
   const size_t lds = kpass_lds_bytes<T, SPIN, N, Q, TX, MULTI>();
   if (lds > h->lds_max) return fail(FGPE_EINVAL, "k pass needs %zu B of LDS", lds);
