@@ -172,7 +172,10 @@ constexpr int SQX32[FGPE_BATCH_NSCHED] = {4, 8, 4, 4, 12, 6, 4, 4};
 constexpr int SQK32[FGPE_BATCH_NSCHED] = {8, 4, 8, 4, 12, 6, 8, 4};
 constexpr int SQX64[FGPE_BATCH_NSCHED] = {8, 4, 8, 4, 6, 12, 8, 4};
 constexpr int SQK64[FGPE_BATCH_NSCHED] = {4, 8, 4, 8, 6, 6, 4, 8};
-constexpr int BTX = 8;  // x columns per k-pass block
+// x columns per k-pass block: whole 128 B rows in FP64 (8); 16 in FP32 (the
+// square spin-1 sweep 981k -> 1,026k instance-steps/s against 8, the others
+// within 2 %; 16 in FP64 made the FP64 solve of the sweep 2.5 % slower)
+template <typename T> constexpr int btx() { return sizeof(T) == 4 ? 16 : 8; }
 int size_class(int n) { return n == 64 ? 0 : n == 128 ? 1 : n == 192 ? 2 : n == 256 ? 3 : -1; }
 template <typename T> int sqx(int s) { return sizeof(T) == 4 ? SQX32[s] : SQX64[s]; }
 template <typename T> int sqk(int s) { return sizeof(T) == 4 ? SQK32[s] : SQK64[s]; }
@@ -419,6 +422,7 @@ int xlaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int ny, int flags
 // (nx columns each)
 template <typename T, int SPIN, int NY, int Q> int klaunch(fgpe_batch* b, hipStream_t st, int start, int cnt, int nx) {
   constexpr int P = NY / Q;
+  constexpr int BTX = btx<T>();
   static_assert(BTX * P <= 1024, "batch k pass block shape");
   const int nb = nx / BTX;
   const size_t lds = kpass_lds_bytes<T, SPIN, NY, Q, BTX>();
